@@ -1,0 +1,129 @@
+// common.hpp -- definitions shared by the HIP kernels and the host runtime.
+//
+// Exact grid costs. A path with `a` orthogonal and `b` diagonal steps costs
+// a + b*sqrt(2) (utils.hpp:119-122 of uAstar uses FP32 for this). Here a cost is
+// the integer key
+//      key(a, b) = a * 2^36 + b * S,   S = 97184015999 = floor(sqrt(2) * 2^36) (odd)
+// which is linear, so path keys add up exactly, step by step, in any order.
+// For two costs with values below COST_LIMIT = 180000 the key order equals the
+// exact order of a + b*sqrt(2): distinct values differ by at least 1/(2F), the
+// key error is below b * 2^-36 <= F * 2^-36, and 2F^2 < 2^36 for F <= 185363.
+// Because S is odd, (a, b) is recovered from the key modulo 2^36.
+#pragma once
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#define UA_HD __host__ __device__
+#else
+#define UA_HD
+#endif
+
+namespace ua {
+
+constexpr int WAVE = 64;
+constexpr uint64_t KEY_ONE = 1ull << 36;
+constexpr uint64_t KEY_SQRT2 = 97184015999ull;
+constexpr uint64_t KEY_MASK36 = (1ull << 36) - 1;
+constexpr double COST_LIMIT = 180000.0;
+constexpr uint64_t KEY_LIMIT = 180000ull << 36;   // every f key must stay below this
+constexpr uint64_t SENT = ~0ull;                  // empty heap slot / "infinity"
+
+// Move table of uAstar (utils.hpp:120-121): 0..3 orthogonal, 4..7 diagonal.
+// The same order serves the puzzle's blank moves (indices 0..3).
+__constant__ static const int kDX[8] = {1, -1, 0, 0, 1, 1, -1, -1};
+__constant__ static const int kDY[8] = {0, 0, 1, -1, 1, -1, 1, -1};
+
+UA_HD inline uint64_t mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+// Grid generator spec: cell id (x*W+y) is blocked iff (cell_hash(seed,id) >> 32) < thr,
+// thr = floor(block_rate * 2^32); start and goal are forced open.
+UA_HD inline uint64_t cell_hash(uint64_t seed_mixed, uint64_t id) {
+    return mix64(seed_mixed + id * 0x9E3779B97F4A7C15ull);
+}
+
+UA_HD inline uint64_t cost_key(uint64_t a, uint64_t b) { return a * KEY_ONE + b * KEY_SQRT2; }
+UA_HD inline uint64_t octile_key(int x, int y, int ex, int ey) {
+    int dx = x > ex ? x - ex : ex - x;
+    int dy = y > ey ? y - ey : ey - y;
+    int mn = dx < dy ? dx : dy;
+    int df = dx > dy ? dx - dy : dy - dx;
+    return cost_key((uint64_t)df, (uint64_t)mn);
+}
+// inverse of S modulo 2^36 (S odd), by Newton iteration
+UA_HD inline uint64_t inv_sqrt2_mod36() {
+    uint64_t x = KEY_SQRT2;  // correct to 3 bits
+    for (int i = 0; i < 6; ++i) x *= 2 - KEY_SQRT2 * x;
+    return x & KEY_MASK36;
+}
+UA_HD inline void key_to_pair(uint64_t key, int64_t *a, int64_t *b) {
+    uint64_t bb = ((key & KEY_MASK36) * inv_sqrt2_mod36()) & KEY_MASK36;
+    *b = (int64_t)bb;
+    *a = (int64_t)((key - bb * KEY_SQRT2) >> 36);
+}
+
+// ------------------------------------------------------------------ query records
+enum : int32_t { KIND_GRID = 1, KIND_PUZZLE = 2 };
+
+// Status codes, identical to include/uastar.h
+enum : int32_t {
+    ST_OK = 0,
+    ST_NO_PATH = 1,
+    ST_INVALID = -1,
+    ST_OOM = -2,
+    ST_CAPACITY = -3,
+    ST_RANGE = -4,
+    ST_HIP = -5,
+    ST_NO_DEVICE = -6,
+    ST_ROUNDS = -7,
+    ST_NO_PDB = -8,
+    ST_ABORT = -9,
+};
+// error bits raised by the kernels
+enum : uint32_t { FLAG_CAPACITY = 1, FLAG_RANGE = 2, FLAG_ABORT = 4 };
+
+// One search instance as the kernels see it. All pointers are device pointers.
+struct QueryDesc {
+    int32_t kind;
+    int32_t n;               // puzzle side
+    int32_t H, W;            // grid
+    int32_t sx, sy, ex, ey;
+    int32_t wpr;             // bitmap words per row
+    int32_t ybits, cbits, qshift;  // grid item encoding
+    int32_t heap_cap;        // full heap nodes per heap (root included)
+    int32_t hash_log2;       // puzzle hash table size
+    int32_t max_rounds;
+    int32_t heaps;           // heaps (= waves) in the team; filled by the launcher
+    const uint64_t *bitmap;  // grid: row-padded, 1 = open
+    uint64_t *best;          // grid: per cell (gkey << 4 | dir);  puzzle: hash keys
+    uint32_t *vals;          // puzzle: hash values (g << 4 | dir)
+    uint64_t *heap;          // heap node storage, heaps * (heap_cap-1) * 64 u64
+    uint64_t start_state;    // puzzle
+};
+
+struct QueryResult {
+    int32_t status;
+    int32_t found;
+    uint64_t goal;       // grid: goal g key; puzzle: (g << 32) | goal slot
+    int64_t rounds;
+    int64_t pops, expanded, generated, inserted;
+    uint32_t flags;
+    int32_t pad;
+    int64_t t_start, t_end;   // device clock (s_memrealtime, 100 MHz) for latency
+};
+
+// Per-team scratch that lives across rounds (global memory).
+struct TeamVars {
+    uint64_t lb[2];          // min popped item key per round slot
+    uint32_t seg_cnt[2];     // insert-buffer fill per round slot
+    uint64_t goal;           // min goal key so far
+    uint32_t flags;
+    uint32_t bar_count;
+    uint32_t bar_gen;
+    uint32_t pad;
+    uint64_t n_pop, n_exp, n_gen, n_ins;
+};
+
+}  // namespace ua

@@ -1,0 +1,225 @@
+// wave_heap.hpp -- wave64 primitives and the per-wavefront batched binary heap.
+//
+// The open list of each heap is a binary heap whose nodes are sorted batches of
+// 64 keys (one key per lane, coalesced 512-byte loads and stores). The top of the
+// heap -- the root batch and a partial batch of fewer than 64 keys -- stays in
+// registers of the owning wavefront; nodes 2..n live in HBM. Invariants:
+//   (H1) for every node v and child c: max(v) <= min(c)
+//   (H2) every key of the partial batch is >= max(root)
+// so the root holds the 64 smallest keys of the heap and a pop returns them.
+// All merges are wave-wide bitonic networks over registers (ds_bpermute).
+// This replaces uAstar's one-heap-per-thread sift-down/sift-up in global memory
+// (pathway/GPU-kernel.cuh:209-252 and :517-535; puzzle/GPU-kernel.cuh:262-298, :435-453).
+#pragma once
+#include <hip/hip_runtime.h>
+#include "common.hpp"
+
+namespace ua {
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint64_t shfl64(uint64_t v, int src) {
+    int lo = __shfl((int)(uint32_t)v, src);
+    int hi = __shfl((int)(uint32_t)(v >> 32), src);
+    return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
+}
+__device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int m) {
+    int lo = __shfl_xor((int)(uint32_t)v, m);
+    int hi = __shfl_xor((int)(uint32_t)(v >> 32), m);
+    return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
+}
+__device__ __forceinline__ uint64_t readlane64(uint64_t v, int lane) {
+    uint32_t lo = __builtin_amdgcn_readlane((int)(uint32_t)v, lane);
+    uint32_t hi = __builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), lane);
+    return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ uint64_t umin64(uint64_t a, uint64_t b) { return a < b ? a : b; }
+__device__ __forceinline__ uint64_t umax64(uint64_t a, uint64_t b) { return a < b ? b : a; }
+
+// Ascending bitonic sort of one key per lane.
+__device__ __forceinline__ uint64_t wave_sort(uint64_t v) {
+    const int l = lane_id();
+#pragma unroll
+    for (int k = 2; k <= 64; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            uint64_t o = shfl_xor64(v, j);
+            bool up = (l & k) == 0;
+            bool lower = (l & j) == 0;
+            v = (lower == up) ? umin64(v, o) : umax64(v, o);
+        }
+    }
+    return v;
+}
+
+// In: a, b ascending. Out: a = the 64 smallest of a U b, b = the 64 largest, both ascending.
+__device__ __forceinline__ void wave_merge(uint64_t &a, uint64_t &b) {
+    const int l = lane_id();
+    uint64_t br = shfl64(b, 63 - l);
+    uint64_t lo = umin64(a, br), hi = umax64(a, br);
+#pragma unroll
+    for (int j = 32; j > 0; j >>= 1) {
+        uint64_t o1 = shfl_xor64(lo, j), o2 = shfl_xor64(hi, j);
+        bool upper = (l & j) != 0;
+        lo = upper ? umax64(lo, o1) : umin64(lo, o1);
+        hi = upper ? umax64(hi, o2) : umin64(hi, o2);
+    }
+    a = lo;
+    b = hi;
+}
+
+// Exclusive prefix sum of a per-lane count; returns the wave total in *total.
+__device__ __forceinline__ int wave_excl_scan(int c, int *total) {
+    const int l = lane_id();
+    int s = c;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        int o = __shfl_up(s, d);
+        if (l >= d) s += o;
+    }
+    *total = __builtin_amdgcn_readlane(s, 63);
+    return s - c;
+}
+
+__device__ __forceinline__ uint64_t wave_min64(uint64_t v) {
+#pragma unroll
+    for (int j = 32; j > 0; j >>= 1) v = umin64(v, shfl_xor64(v, j));
+    return readlane64(v, 0);
+}
+__device__ __forceinline__ int64_t wave_sum64(int64_t v) {
+#pragma unroll
+    for (int j = 32; j > 0; j >>= 1) v += (int64_t)shfl_xor64((uint64_t)v, j);
+    return (int64_t)readlane64((uint64_t)v, 0);
+}
+
+// ------------------------------------------------------------------ the heap
+struct WaveHeap {
+    uint64_t root;    // this lane's key of the root batch
+    uint64_t part;    // this lane's key of the partial batch (SENT beyond pc)
+    int n;            // full nodes, root included (wave-uniform)
+    int pc;           // keys in the partial batch (wave-uniform)
+    int cap;          // max full nodes, root included
+    uint64_t *nodes;  // node i >= 2 at nodes + (i - 2) * 64
+
+    __device__ void init(uint64_t *storage, int capacity) {
+        root = SENT;
+        part = SENT;
+        n = 0;
+        pc = 0;
+        cap = capacity;
+        nodes = storage;
+    }
+    __device__ __forceinline__ uint64_t load(int i) const {
+        return nodes[(size_t)(i - 2) * 64 + lane_id()];
+    }
+    __device__ __forceinline__ void store(int i, uint64_t v) const {
+        nodes[(size_t)(i - 2) * 64 + lane_id()] = v;
+    }
+    __device__ __forceinline__ uint64_t min_key() const {
+        return n >= 1 ? readlane64(root, 0) : readlane64(part, 0);
+    }
+    __device__ __forceinline__ int size() const { return n * 64 + pc; }
+
+    // Restore (H1) below the root after the root batch changed.
+    __device__ void sift_down() {
+        int v = 1;
+        uint64_t X = root;
+        while (true) {
+            int l = 2 * v;
+            if (l > n) break;
+            uint64_t L = load(l);
+            uint64_t xmax = readlane64(X, 63);
+            if (l + 1 > n) {
+                if (xmax <= readlane64(L, 0)) break;
+                wave_merge(X, L);
+                store(l, L);
+                break;
+            }
+            uint64_t R = load(l + 1);
+            if (xmax <= readlane64(L, 0) && xmax <= readlane64(R, 0)) break;
+            uint64_t lmax = readlane64(L, 63), rmax = readlane64(R, 63);
+            wave_merge(L, R);                       // L = smaller half, R = larger half
+            int big = (lmax >= rmax) ? l : l + 1;   // larger half replaces the child with the larger max
+            int small = (big == l) ? l + 1 : l;
+            store(big, R);
+            wave_merge(X, L);                       // X keeps the 64 smallest
+            if (v == 1) root = X;
+            else store(v, X);
+            v = small;
+            X = L;
+        }
+        if (v == 1) root = X;
+        else store(v, X);
+    }
+
+    // Pops the (up to) 64 smallest keys, one per lane, ascending; SENT where empty.
+    __device__ uint64_t pop() {
+        uint64_t out;
+        if (n >= 1) {
+            out = root;
+            if (n >= 2) {
+                root = load(n);
+                --n;
+                wave_merge(root, part);  // root takes 64 of (last node U partial), partial keeps pc
+                sift_down();
+            } else {
+                n = 0;
+                root = SENT;
+            }
+        } else {
+            out = part;
+            part = SENT;
+            pc = 0;
+        }
+        return out;
+    }
+
+    // Inserts a full batch F whose keys are all >= max(root), as a new leaf,
+    // merging top-down along the path (the root is skipped). Returns false on overflow.
+    __device__ bool push_full(uint64_t F) {
+        int target = n + 1;
+        if (target > cap) return false;
+        int depth = 31 - __builtin_clz((unsigned)target);
+        for (int L = 1; L < depth; ++L) {
+            int v = target >> (depth - L);
+            uint64_t X = load(v);
+            if (readlane64(X, 63) <= readlane64(F, 0)) continue;
+            wave_merge(X, F);
+            store(v, X);
+        }
+        store(target, F);
+        n = target;
+        return true;
+    }
+
+    // Inserts s (<= 64) keys S, ascending with SENT padding. Returns false on overflow.
+    __device__ bool insert(uint64_t S, int s) {
+        if (s <= 0) return true;
+        int t = pc + s;
+        if (n >= 1) {
+            wave_merge(root, S);        // root keeps the 64 smallest, S the s largest
+            wave_merge(part, S);        // part = smallest min(64,t), S = the rest
+            if (t < 64) {
+                pc = t;
+            } else {
+                uint64_t F = part;      // 64 keys, all >= max(root)
+                part = S;
+                pc = t - 64;
+                return push_full(F);
+            }
+        } else {
+            wave_merge(part, S);
+            if (t < 64) {
+                pc = t;
+            } else {
+                root = part;
+                part = S;
+                pc = t - 64;
+                n = 1;
+            }
+        }
+        return true;
+    }
+};
+
+}  // namespace ua
