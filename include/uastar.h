@@ -1,0 +1,137 @@
+/*
+ * uastar.h -- C ABI of libuastar, the MI355X (gfx950) GA* engine.
+ *
+ * Plain pointers and sizes only. Every entry point returns a status code
+ * (UA_OK = 0, UA_NO_PATH = 1, negative = error; see ua_last_error()).
+ * Citations name the uAstar interface each entry point replaces (paths relative
+ * to the uAstar source tree).
+ *
+ * Grid bitmaps: row-padded, 1 bit per cell, 1 = open. Row x occupies words
+ * [x*wpr, (x+1)*wpr) with wpr = (W + 63) / 64; cell (x, y) is bit (y & 63) of word
+ * x*wpr + (y >> 6). Cell ids are x*W + y (pathway.hpp:99-108). A move u->v is legal
+ * when u is open and v is in range; a blocked v is a dead end and may only be the
+ * goal (pathway/CPU-solver.cpp:64-70 with all-or-nothing cell masks).
+ * Costs are exact: orth + diag*sqrt(2) with integer step counts.
+ */
+#ifndef UASTAR_H
+#define UASTAR_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define UA_OK 0
+#define UA_NO_PATH 1          /* search exhausted without reaching the goal */
+#define UA_ERR_INVALID (-1)   /* bad argument */
+#define UA_ERR_OOM (-2)       /* device allocation failed / over the memory budget */
+#define UA_ERR_CAPACITY (-3)  /* heap or hash capacity exhausted (retries exhausted) */
+#define UA_ERR_RANGE (-4)     /* cost above the exact-key limit (180000) */
+#define UA_ERR_HIP (-5)       /* HIP runtime error */
+#define UA_ERR_NO_DEVICE (-6) /* no usable GPU */
+#define UA_ERR_ROUNDS (-7)    /* round limit reached */
+#define UA_ERR_NO_PDB (-8)    /* puzzle solve before ua_puzzle_build_pdb */
+#define UA_ERR_ABORT (-9)     /* a device barrier timed out; results discarded */
+
+#define UA_QUERY_GRID 1
+#define UA_QUERY_PUZZLE 2
+
+typedef struct ua_ctx ua_ctx;
+typedef struct ua_grid ua_grid; /* a device-resident grid bitmap */
+
+typedef struct {
+    int32_t team_waves;  /* waves (= heaps) per query in batch/serve; 0 = autotuned */
+    int32_t plain_heaps; /* heaps (= waves) of the plain whole-GPU solve; 0 = one per CU */
+    int32_t max_rounds;  /* 0 = 1<<26 */
+    int32_t hash_log2;   /* puzzle hash table size per query; 0 = autotuned */
+    uint64_t mem_budget; /* device bytes the batch/serve planner may use; 0 = 85% of free */
+} ua_config;
+
+typedef struct {
+    int32_t status;
+    int32_t found;
+    int64_t orth, diag;  /* grid: exact cost = orth + diag*sqrt(2) */
+    int64_t steps;       /* puzzle: optimal number of moves */
+    double cost;         /* grid: orth + diag*sqrt(2) as a double */
+    int64_t rounds, pops, expanded, generated, inserted;
+    double device_ms;    /* search time on the device (s_memrealtime stamps of the team) */
+    double kernel_ms;    /* duration of the search launch, from HIP events on the context's stream
+                            (a batch: the launch that ran this query, shared by its sub-batch) */
+    double latency_ms;   /* serve: completion - submission; batch: completion - launch */
+} ua_result;
+
+/* Device context: owns every device allocation. Replaces the device set-up of
+ * GPUPathwaySolver::initialize (pathway/GPU-solver.cu:91-140) and
+ * GPUPuzzleSolver<N>::initialize (puzzle/GPU-solver.cuh:64-195); no device reset. */
+int ua_create(int ordinal, const ua_config *cfg, ua_ctx **out);
+void ua_destroy(ua_ctx *ctx);
+const char *ua_last_error(ua_ctx *ctx);
+/* 1 if a GPU is visible, without creating a context */
+int ua_device_count(void);
+
+/* ---- grids. Replace the H2D copy of Pathway::graph() (pathway/GPU-solver.cu:140). */
+int ua_grid_create(ua_ctx *ctx, int H, int W, const uint64_t *host_bitmap, ua_grid **out);
+/* from the byte-per-cell graph of Pathway::graph() (0 = blocked, nonzero = open) */
+int ua_grid_from_bytes(ua_ctx *ctx, int H, int W, const uint8_t *bytes, ua_grid **out);
+/* i.i.d. Bernoulli(block_rate) obstacles from seed, generated on the device;
+ * start and goal forced open (the advertised `random` input module, main.cpp:279-281) */
+int ua_grid_generate(ua_ctx *ctx, int H, int W, uint64_t seed, double block_rate, int sx, int sy,
+                     int ex, int ey, ua_grid **out);
+int ua_grid_download(ua_ctx *ctx, const ua_grid *g, uint64_t *host_bitmap);
+void ua_grid_free(ua_ctx *ctx, ua_grid *g);
+
+/* Plain single-query solve on the whole GPU: replaces GPUPathwaySolver::solve
+ * (pathway/GPU-solver.cu:142-290) -> UA_OK (found) or UA_NO_PATH. */
+int ua_grid_solve(ua_ctx *ctx, const ua_grid *g, int sx, int sy, int ex, int ey, ua_result *res);
+/* Path of the last ua_grid_solve, start..goal cell ids: replaces
+ * GPUPathwaySolver::getSolution (pathway/GPU-solver.cu:292-316). */
+int ua_grid_path(ua_ctx *ctx, int32_t *ids, int64_t cap, int64_t *len);
+
+/* ---- sliding-tile puzzles, n = 3 or 4. */
+/* Builds the reference's additive disjoint PDB on the device (puzzle/database.cpp:37-84;
+ * partitions of puzzle/CPU-solver.hpp:44-70); replaces the PDB upload of
+ * GPUPuzzleSolver<N>::initialize (puzzle/GPU-solver.cuh:130-160). */
+int ua_puzzle_build_pdb(ua_ctx *ctx, int n);
+/* Copies the device PDB (all partitions concatenated) to the host. */
+int ua_puzzle_download_pdb(ua_ctx *ctx, int n, uint8_t *host, uint64_t size);
+/* Plain solve of one board (row-major, 0 = blank): replaces GPUPuzzleSolver<N>::solve
+ * (puzzle/GPU-solver.cuh:197-274). Unsolvable boards return UA_NO_PATH at once. */
+int ua_puzzle_solve(ua_ctx *ctx, int n, const uint8_t *tiles, ua_result *res);
+/* Blank moves of the last ua_puzzle_solve (0..3 = (+1,0),(-1,0),(0,+1),(0,-1)):
+ * replaces GPUPuzzleSolver<N>::getSolution (puzzle/GPU-solver.cuh:276-287). */
+int ua_puzzle_moves(ua_ctx *ctx, int32_t *dirs, int64_t cap, int64_t *len);
+
+/* ---- batching: many uneven instances in one persistent launch (no reference counterpart;
+ * it generalises the per-query loop of main.cpp:60-110 to many queries at once). */
+typedef struct {
+    int32_t kind;            /* UA_QUERY_GRID / UA_QUERY_PUZZLE */
+    int32_t n;               /* puzzle side */
+    const ua_grid *grid;     /* grid queries */
+    int32_t sx, sy, ex, ey;
+    uint8_t tiles[16];       /* puzzle board */
+} ua_query;
+int ua_batch_solve(ua_ctx *ctx, const ua_query *qs, int64_t nq, ua_result *out);
+/* path / moves of query i of the last ua_batch_solve (while its state is resident) */
+int ua_batch_path(ua_ctx *ctx, int64_t i, int32_t *out, int64_t cap, int64_t *len);
+
+/* ---- serving: continuous batching (no reference counterpart). Submitted queries are
+ * admitted as team slots and memory free up; ua_serve_step() advances the device by one
+ * bounded launch and retires finished queries. Submit/poll are thread-safe. */
+int ua_serve_submit(ua_ctx *ctx, const ua_query *q, uint64_t *ticket);
+int ua_serve_step(ua_ctx *ctx, int64_t *n_active, int64_t *n_pending);
+/* up to cap finished queries: their tickets and results */
+int ua_serve_poll(ua_ctx *ctx, uint64_t *tickets, ua_result *results, int64_t cap, int64_t *n_out);
+/* steps until nothing is pending or active */
+int ua_serve_drain(ua_ctx *ctx);
+
+/* ---- diagnostics */
+/* Runs a random insert/pop sequence on one wave's heap; see tests. */
+int ua_debug_heap_selftest(ua_ctx *ctx, const int32_t *ops, int nops, const uint64_t *keys, int64_t nkeys,
+                           int cap_nodes, uint64_t *out_pops, int64_t out_cap, int32_t *final_size);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,130 @@
+"""CPU run of the device search code (uastar_amd/csrc/search.hpp) under the SIMT
+emulator in tests/emu/: one workgroup team, host memory, results against the oracle.
+
+This checks the batch kernel's logic (workgroup teams, LDS round exchange,
+per-wave heaps with HBM levels) without a GPU.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import uastar_amd as U
+from tests.helpers import goal_board, grid_bytes, random_grid
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+EMU = os.path.join(HERE, "emu")
+LIB = os.path.join(EMU, "libemu.so")
+CSRC = os.path.join(os.path.dirname(HERE), "uastar_amd", "csrc")
+
+
+class QueryDesc(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("kind", "n", "H", "W", "sx", "sy", "ex", "ey", "wpr", "ybits", "cbits",
+                                          "qshift", "heap_cap", "hash_log2", "max_rounds", "heaps")] + [
+        ("bitmap", C.c_void_p), ("best", C.c_void_p), ("vals", C.c_void_p), ("heap", C.c_void_p),
+        ("start_state", C.c_uint64)]
+
+
+class QueryResult(C.Structure):
+    _fields_ = [("status", C.c_int32), ("found", C.c_int32), ("goal", C.c_uint64), ("rounds", C.c_int64),
+                ("pops", C.c_int64), ("expanded", C.c_int64), ("generated", C.c_int64), ("inserted", C.c_int64),
+                ("flags", C.c_uint32), ("pad", C.c_int32), ("t_start", C.c_int64), ("t_end", C.c_int64)]
+
+
+@pytest.fixture(scope="module")
+def emu():
+    srcs = [os.path.join(EMU, "emu_batch.cpp"), os.path.join(EMU, "hip_emu.hpp")] + [
+        os.path.join(CSRC, f) for f in ("search.hpp", "wave_heap.hpp", "common.hpp")]
+    if not os.path.exists(LIB) or os.path.getmtime(LIB) < max(os.path.getmtime(s) for s in srcs):
+        subprocess.check_call(["g++", "-O2", "-std=c++20", "-fPIC", "-shared", "-pthread", "-I", CSRC,
+                               srcs[0], "-o", LIB + ".tmp"])
+        os.replace(LIB + ".tmp", LIB)
+    L = C.CDLL(LIB)
+    L.emu_batch.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
+    return L
+
+
+def bits_for(v):
+    b = 1
+    while (1 << b) < v:
+        b += 1
+    return b
+
+
+class Batch:
+    """Host-memory version of the runtime's batch planning (best tables, heaps)."""
+
+    def __init__(self, waves, max_rounds=100_000):
+        self.waves, self.max_rounds = waves, max_rounds
+        self.descs, self.keep = [], []
+
+    def add_grid(self, graph, H, W, sx, sy, ex, ey, items=65536):
+        bm = U.bytes_to_bitmap(graph, H, W)
+        best = np.full(H * W, np.uint64(~np.uint64(0)))
+        per_heap = (items + 64 * self.waves - 1) // (64 * self.waves)
+        cap = max(per_heap + 1, 4)
+        heap = np.zeros(self.waves * (cap - 1) * 64, np.uint64)
+        d = QueryDesc(kind=1, H=H, W=W, sx=sx, sy=sy, ex=ex, ey=ey, wpr=(W + 63) // 64, ybits=bits_for(W))
+        d.cbits = bits_for(H) + d.ybits
+        d.qshift = max(0, d.cbits - 10)
+        d.heap_cap, d.max_rounds, d.heaps = cap, self.max_rounds, self.waves
+        d.bitmap, d.best, d.heap = bm.ctypes.data, best.ctypes.data, heap.ctypes.data
+        self.keep += [bm, best, heap]
+        self.descs.append(d)
+
+    def add_puzzle(self, n, tiles, hash_log2=17, items=65536):
+        keys = np.zeros(1 << hash_log2, np.uint64)
+        vals = np.full(1 << hash_log2, 0xFFFFFFFF, np.uint32)
+        per_heap = (items + 64 * self.waves - 1) // (64 * self.waves)
+        cap = max(per_heap + 1, 4)
+        heap = np.zeros(self.waves * (cap - 1) * 64, np.uint64)
+        s = 0
+        for i, t in enumerate(tiles):
+            s |= int(t) << (4 * i)
+        d = QueryDesc(kind=2, n=n, hash_log2=hash_log2, heap_cap=cap, max_rounds=self.max_rounds, heaps=self.waves)
+        d.best, d.vals, d.heap, d.start_state = keys.ctypes.data, vals.ctypes.data, heap.ctypes.data, s
+        self.keep += [keys, vals, heap]
+        self.descs.append(d)
+
+    def run(self, lib, pdb3=None, pdb4=None):
+        arr = (QueryDesc * len(self.descs))(*self.descs)
+        res = (QueryResult * len(self.descs))()
+        p3 = pdb3.ctypes.data if pdb3 is not None else None
+        p4 = pdb4.ctypes.data if pdb4 is not None else None
+        assert lib.emu_batch(arr, res, len(self.descs), self.waves, p3, p4) == 0
+        return list(res)
+
+
+def test_emulated_batch_grids(emu, oracle):
+    rng = np.random.default_rng(42)
+    b = Batch(waves=4)
+    exp = []
+    for i in range(6):
+        H, W = int(rng.integers(2, 60)), int(rng.integers(2, 60))
+        m = random_grid(rng, H, W, float(rng.uniform(0.0, 0.4)))
+        sx, sy, ex, ey = (int(rng.integers(H)), int(rng.integers(W)), int(rng.integers(H)), int(rng.integers(W)))
+        m[sx, sy] = True
+        g = grid_bytes(m)
+        b.add_grid(g, H, W, sx, sy, ex, ey, items=4096)   # small heaps: HBM levels are used
+        exp.append(oracle.grid_astar(g, H, W, sx, sy, ex, ey, "ref"))
+    res = b.run(emu)
+    for i, (r, e) in enumerate(zip(res, exp)):
+        assert r.status in (0, 1), (i, r.status)
+        assert bool(r.found) == e["found"], i
+        if e["found"]:
+            assert U.key_to_pair(r.goal) == (e["orth"], e["diag"]), i
+
+
+def test_emulated_batch_puzzles(emu, oracle):
+    pdb3 = oracle.pdb_full(3)
+    b = Batch(waves=2)
+    exp = []
+    for s in range(4):
+        t = oracle.puzzle_random(3, 900 + s)
+        b.add_puzzle(3, t)
+        exp.append(oracle.puzzle_solve(3, t, pdb3, "astar")[0])
+    res = b.run(emu, pdb3=pdb3)
+    for r, e in zip(res, exp):
+        assert r.status == 0 and r.found and (r.goal >> 32) == e

@@ -114,14 +114,21 @@ struct QueryResult {
     int64_t t_start, t_end;   // device clock (s_memrealtime, 100 MHz) for latency
 };
 
+// Per-round values of a team. Round r uses slot r & 1; the barrier that ends
+// round r resets slot (r + 1) & 1, so every wave reads identical values.
+struct RoundSlot {
+    uint64_t lb;     // min popped item key of the round (SENT = every heap was empty)
+    uint64_t goal;   // min goal key found in the round
+    uint32_t cnt;    // items appended to the insert buffer
+    uint32_t err;    // FLAG_* raised between the previous barrier and this one
+};
+
 // Per-team scratch that lives across rounds (global memory).
 struct TeamVars {
-    uint64_t lb[2];          // min popped item key per round slot
-    uint32_t seg_cnt[2];     // insert-buffer fill per round slot
-    uint64_t goal;           // min goal key so far
-    uint32_t flags;
+    RoundSlot slot[2];
     uint32_t bar_count;
     uint32_t bar_gen;
+    uint32_t abort;
     uint32_t pad;
     uint64_t n_pop, n_exp, n_gen, n_ins;
 };

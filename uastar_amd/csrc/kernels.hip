@@ -1,0 +1,325 @@
+// kernels.hip -- the GA* search kernels for gfx950 and their launchers. The team
+// search itself (rounds, teams, grid and puzzle expansion) is in search.hpp.
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+#include "kernels.hpp"
+#include "search.hpp"
+#include "wave_heap.hpp"
+
+namespace ua {
+
+// Persistent batch kernel: each workgroup is a team and pulls query indices from
+// a device counter until the list is exhausted (no spin waits).
+__global__ __launch_bounds__(512) void k_batch(const QueryDesc *qs, QueryResult *rs, int nq,
+                                               uint32_t *qhead, int segcap, const uint8_t *pdb3,
+                                               const uint8_t *pdb4) {
+    extern __shared__ uint64_t lds[];
+    batch_block(qs, rs, nq, qhead, segcap, pdb3, pdb4, lds);
+}
+
+// Plain solve: one query, every wave of a cooperative launch is one team.
+__global__ __launch_bounds__(256) void k_wide(const QueryDesc *qs, QueryResult *rs, TeamVars *tv,
+                                              uint64_t *seg, int segcap, const uint8_t *pdb3,
+                                              const uint8_t *pdb4) {
+    TeamMem m;
+    m.slot = tv->slot;
+    m.stats = &tv->n_pop;
+    m.seg = seg;
+    m.segcap = segcap;
+    GridTeam team(m, tv);
+    int64_t t0 = now_ticks();
+    QueryDesc q = qs[0];
+    dispatch(q, rs, pdb3, pdb4, team);
+    if (team.leader()) {
+        rs->t_start = t0;
+        rs->t_end = now_ticks();
+    }
+}
+
+// ------------------------------------------------------------------ path walks
+// Single lane: walks the parent directions from the goal (pathway/GPU-kernel.cuh:546-569
+// walks node records instead). ids receive goal..start; *len = count or -1 on a broken chain.
+__global__ void k_grid_path(const uint64_t *best, int H, int W, int sx, int sy, int ex, int ey,
+                            int32_t *ids, int64_t cap, int64_t *len) {
+    if (threadIdx.x || blockIdx.x) return;
+    int x = ex, y = ey;
+    int64_t c = 0;
+    while (c < cap) {
+        ids[c++] = x * W + y;
+        if (x == sx && y == sy) {
+            *len = c;
+            return;
+        }
+        int d = (int)(best[(size_t)x * W + y] & 15ull);
+        if (d > 7) break;
+        x -= kDX[d];
+        y -= kDY[d];
+        if (x < 0 || x >= H || y < 0 || y >= W) break;
+    }
+    *len = -1;
+}
+
+// Single lane: blank-move indices goal..start (puzzle/GPU-kernel.cuh:462-509 semantics).
+template <int N>
+__global__ void k_puzzle_moves(const uint64_t *keys, const uint32_t *vals, int hash_log2,
+                               uint64_t goal_slot, int32_t *dirs, int64_t cap, int64_t *len) {
+    if (threadIdx.x || blockIdx.x) return;
+    uint64_t hmask = (1ull << hash_log2) - 1;
+    uint64_t slot = goal_slot;
+    uint64_t b = keys[slot];
+    int64_t c = 0;
+    while (c <= cap) {
+        int d = (int)(vals[slot] & 15u);
+        if (d == 15) {
+            *len = c;
+            return;
+        }
+        if (d > 3 || c == cap) break;
+        dirs[c++] = d;
+        int bl = nib_pos(b, 0);
+        int px = bl / N - kDX[d], py = bl % N - kDY[d];
+        if (px < 0 || px >= N || py < 0 || py >= N) break;
+        int pb = px * N + py;
+        uint64_t t = (b >> (4 * pb)) & 15ull;
+        b = b + (t << (4 * bl)) - (t << (4 * pb));
+        slot = mix64(b) & hmask;
+        int i = 0;
+        for (; i < PuzzleProb<N>::MAXPROBE && keys[slot] != b; ++i) slot = (slot + 1) & hmask;
+        if (i == PuzzleProb<N>::MAXPROBE) break;
+    }
+    *len = -1;
+}
+
+// ------------------------------------------------------------------ grid generator
+// One thread per bitmap word; see cell_hash() for the spec.
+__global__ void k_grid_gen(uint64_t *bm, int H, int W, int wpr, uint64_t seed_mixed, uint64_t thr,
+                           int sx, int sy, int ex, int ey) {
+    size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= (size_t)H * wpr) return;
+    int x = (int)(w / wpr), y0 = (int)(w % wpr) * 64;
+    uint64_t word = 0;
+    for (int b = 0; b < 64; ++b) {
+        int y = y0 + b;
+        if (y >= W) break;
+        bool open = (cell_hash(seed_mixed, (uint64_t)x * W + y) >> 32) >= thr;
+        if ((x == sx && y == sy) || (x == ex && y == ey)) open = true;
+        word |= (uint64_t)open << b;
+    }
+    bm[w] = word;
+}
+
+// Byte-per-cell input (pathway.hpp:116-119, 0xFF open / 0 blocked) to the bitmap.
+__global__ void k_grid_pack(const uint8_t *bytes, uint64_t *bm, int H, int W, int wpr) {
+    size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= (size_t)H * wpr) return;
+    int x = (int)(w / wpr), y0 = (int)(w % wpr) * 64;
+    uint64_t word = 0;
+    for (int b = 0; b < 64; ++b) {
+        int y = y0 + b;
+        if (y >= W) break;
+        word |= (uint64_t)(bytes[(size_t)x * W + y] != 0) << b;
+    }
+    bm[w] = word;
+}
+
+// ------------------------------------------------------------------ PDB build (device BFS)
+// Level-synchronous BFS over the k-permutation ranks of one partition, from the goal
+// pattern (tile v on cell v-1), a tile moving into any adjacent cell not holding a
+// pattern tile at cost 1 (puzzle/database.cpp:37-84). Each thread scans 16 ranks.
+template <int N, int P>
+__global__ void k_pdb_level(uint8_t *tab, uint64_t size, int d, unsigned long long *added) {
+    using S = PSpec<N>;
+    constexpr int K = S::K[P];
+    constexpr int NN = N * N;
+    uint64_t base = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * 16;
+    unsigned long long cnt = 0;
+    if (base < size) {
+        uint8_t vals[16];
+        if (base + 16 <= size) {
+            uint4 v4 = *reinterpret_cast<const uint4 *>(tab + base);
+            *reinterpret_cast<uint4 *>(vals) = v4;
+        } else {
+            for (int i = 0; i < 16; ++i) vals[i] = base + i < size ? tab[base + i] : 0xFF;
+        }
+        for (int i = 0; i < 16; ++i) {
+            if (vals[i] != d) continue;
+            uint32_t code = (uint32_t)(base + i);
+            int pos[8];
+            uint32_t used = 0;
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                uint32_t v = code / S::M[P][j];
+                code -= v * S::M[P][j];
+                int cell = 0;
+                for (int c = 0; c < NN; ++c) {  // v-th unused cell
+                    if (used >> c & 1u) continue;
+                    if (v == 0) {
+                        cell = c;
+                        break;
+                    }
+                    --v;
+                }
+                pos[j] = cell;
+                used |= 1u << cell;
+            }
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                int x = pos[j] / N, y = pos[j] % N;
+#pragma unroll
+                for (int dd = 0; dd < 4; ++dd) {
+                    int nx = x + kDX[dd], ny = y + kDY[dd];
+                    if (nx < 0 || nx >= N || ny < 0 || ny >= N) continue;
+                    int np = nx * N + ny;
+                    if (used >> np & 1u) continue;
+                    uint32_t mask = 0, nc = 0;
+#pragma unroll
+                    for (int jj = 0; jj < K; ++jj) {
+                        int p = jj == j ? np : pos[jj];
+                        nc += ((uint32_t)p - __builtin_popcount(mask & ((1u << p) - 1u))) * S::M[P][jj];
+                        mask |= 1u << p;
+                    }
+                    if (tab[nc] == 0xFF) {
+                        tab[nc] = (uint8_t)(d + 1);
+                        ++cnt;
+                    }
+                }
+            }
+        }
+    }
+    for (int j = 32; j > 0; j >>= 1) cnt += __shfl_xor(cnt, j);
+    if (lane_id() == 0 && cnt) atomicAdd(added, cnt);
+}
+
+template <int N, int P>
+__global__ void k_pdb_seed(uint8_t *tab) {
+    using S = PSpec<N>;
+    uint32_t mask = 0, code = 0;
+    for (int j = 0; j < S::K[P]; ++j) {
+        int p = S::T[P][j] - 1;
+        code += ((uint32_t)p - __builtin_popcount(mask & ((1u << p) - 1u))) * S::M[P][j];
+        mask |= 1u << p;
+    }
+    tab[code] = 0;
+}
+
+// ------------------------------------------------------------------ heap self-test
+// Random batches inserted / popped on one wave; pops are written out for the host
+// to compare against a sorted reference. ops[i] >= 0: insert keys[k..k+ops[i]);
+// ops[i] < 0: pop. out receives 64 keys per pop.
+__global__ void k_heap_selftest(const int *ops, int nops, const uint64_t *keys, uint64_t *storage,
+                                int cap, uint64_t *out, int *status) {
+    WaveHeap h;
+    h.init(storage, cap);
+    int l = lane_id();
+    int k = 0, o = 0;
+    bool ok = true;
+    for (int i = 0; i < nops; ++i) {
+        int c = ops[i];
+        if (c >= 0) {
+            uint64_t v = l < c ? keys[k + l] : SENT;
+            k += c;
+            v = wave_sort(v);
+            ok &= h.insert(v, c);
+        } else {
+            out[(size_t)o * 64 + l] = h.pop();
+            ++o;
+        }
+    }
+    if (l == 0) *status = ok ? h.size() : -1;
+}
+
+// ------------------------------------------------------------------ launchers
+static int grid_blocks(size_t n, int t) { return (int)((n + t - 1) / t); }
+
+hipError_t launch_grid_gen(uint64_t *bm, int H, int W, int wpr, uint64_t seed, uint64_t thr, int sx,
+                           int sy, int ex, int ey, hipStream_t st) {
+    size_t n = (size_t)H * wpr;
+    hipLaunchKernelGGL(k_grid_gen, dim3(grid_blocks(n, 256)), dim3(256), 0, st, bm, H, W, wpr,
+                       mix64(seed), thr, sx, sy, ex, ey);
+    return hipGetLastError();
+}
+
+hipError_t launch_grid_pack(const uint8_t *bytes, uint64_t *bm, int H, int W, int wpr, hipStream_t st) {
+    size_t n = (size_t)H * wpr;
+    hipLaunchKernelGGL(k_grid_pack, dim3(grid_blocks(n, 256)), dim3(256), 0, st, bytes, bm, H, W, wpr);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch(const QueryDesc *qs, QueryResult *rs, int nq, uint32_t *qhead, int segcap,
+                        const uint8_t *pdb3, const uint8_t *pdb4, int blocks, int waves, hipStream_t st) {
+    hipLaunchKernelGGL(k_batch, dim3(blocks), dim3(waves * 64), wg_lds_bytes(segcap), st, qs, rs, nq, qhead,
+                       segcap, pdb3, pdb4);
+    return hipGetLastError();
+}
+
+hipError_t wide_max_blocks(int *per_cu) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_wide, 256, 0);
+}
+
+hipError_t batch_max_blocks(int waves, int segcap, int *per_cu) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_batch, waves * 64, wg_lds_bytes(segcap));
+}
+
+hipError_t launch_wide(const QueryDesc *qs, QueryResult *rs, TeamVars *tv, uint64_t *seg, int segcap,
+                       const uint8_t *pdb3, const uint8_t *pdb4, int blocks, hipStream_t st) {
+    void *args[] = {(void *)&qs, (void *)&rs, (void *)&tv, (void *)&seg, (void *)&segcap,
+                    (void *)&pdb3, (void *)&pdb4};
+    return hipLaunchCooperativeKernel((const void *)k_wide, dim3(blocks), dim3(256), args, 0, st);
+}
+
+hipError_t launch_grid_path(const uint64_t *best, int H, int W, int sx, int sy, int ex, int ey,
+                            int32_t *ids, int64_t cap, int64_t *len, hipStream_t st) {
+    hipLaunchKernelGGL(k_grid_path, dim3(1), dim3(64), 0, st, best, H, W, sx, sy, ex, ey, ids, cap, len);
+    return hipGetLastError();
+}
+
+hipError_t launch_puzzle_moves(int n, const uint64_t *keys, const uint32_t *vals, int hash_log2,
+                               uint64_t goal_slot, int32_t *dirs, int64_t cap, int64_t *len,
+                               hipStream_t st) {
+    if (n == 4)
+        hipLaunchKernelGGL(k_puzzle_moves<4>, dim3(1), dim3(64), 0, st, keys, vals, hash_log2, goal_slot,
+                           dirs, cap, len);
+    else
+        hipLaunchKernelGGL(k_puzzle_moves<3>, dim3(1), dim3(64), 0, st, keys, vals, hash_log2, goal_slot,
+                           dirs, cap, len);
+    return hipGetLastError();
+}
+
+template <int N, int P>
+static hipError_t pdb_one(uint8_t *tab, uint64_t size, unsigned long long *d_added, hipStream_t st) {
+    hipError_t e = hipMemsetAsync(tab, 0xFF, size, st);
+    if (e) return e;
+    hipLaunchKernelGGL((k_pdb_seed<N, P>), dim3(1), dim3(1), 0, st, tab);
+    for (int d = 0; d < 250; ++d) {
+        e = hipMemsetAsync(d_added, 0, sizeof(*d_added), st);
+        if (e) return e;
+        uint64_t threads = (size + 15) / 16;
+        hipLaunchKernelGGL((k_pdb_level<N, P>), dim3(grid_blocks(threads, 256)), dim3(256), 0, st, tab,
+                           size, d, d_added);
+        unsigned long long h = 0;
+        e = hipMemcpyAsync(&h, d_added, sizeof(h), hipMemcpyDeviceToHost, st);
+        if (e) return e;
+        e = hipStreamSynchronize(st);
+        if (e) return e;
+        if (!h) break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t build_pdb(int n, uint8_t *pdb, unsigned long long *d_added, hipStream_t st) {
+    if (n == 4) {
+        hipError_t e = pdb_one<4, 0>(pdb, PSpec<4>::OFF[1], d_added, st);
+        if (e) return e;
+        return pdb_one<4, 1>(pdb + PSpec<4>::OFF[1], 57657600ull, d_added, st);
+    }
+    return pdb_one<3, 0>(pdb, 362880ull, d_added, st);
+}
+
+hipError_t launch_heap_selftest(const int *ops, int nops, const uint64_t *keys, uint64_t *storage, int cap,
+                                uint64_t *out, int *status, hipStream_t st) {
+    hipLaunchKernelGGL(k_heap_selftest, dim3(1), dim3(64), 0, st, ops, nops, keys, storage, cap, out, status);
+    return hipGetLastError();
+}
+
+}  // namespace ua

@@ -1,0 +1,26 @@
+// kernels.hpp -- host-callable launchers of kernels.hip (internal to libuastar).
+#pragma once
+#include <hip/hip_runtime.h>
+#include "common.hpp"
+
+namespace ua {
+
+hipError_t launch_grid_gen(uint64_t *bm, int H, int W, int wpr, uint64_t seed, uint64_t thr, int sx,
+                           int sy, int ex, int ey, hipStream_t st);
+hipError_t launch_grid_pack(const uint8_t *bytes, uint64_t *bm, int H, int W, int wpr, hipStream_t st);
+hipError_t launch_batch(const QueryDesc *qs, QueryResult *rs, int nq, uint32_t *qhead, int segcap,
+                        const uint8_t *pdb3, const uint8_t *pdb4, int blocks, int waves, hipStream_t st);
+hipError_t wide_max_blocks(int *per_cu);
+hipError_t batch_max_blocks(int waves, int segcap, int *per_cu);
+hipError_t launch_wide(const QueryDesc *qs, QueryResult *rs, TeamVars *tv, uint64_t *seg, int segcap,
+                       const uint8_t *pdb3, const uint8_t *pdb4, int blocks, hipStream_t st);
+hipError_t launch_grid_path(const uint64_t *best, int H, int W, int sx, int sy, int ex, int ey,
+                            int32_t *ids, int64_t cap, int64_t *len, hipStream_t st);
+hipError_t launch_puzzle_moves(int n, const uint64_t *keys, const uint32_t *vals, int hash_log2,
+                               uint64_t goal_slot, int32_t *dirs, int64_t cap, int64_t *len,
+                               hipStream_t st);
+hipError_t build_pdb(int n, uint8_t *pdb, unsigned long long *d_added, hipStream_t st);
+hipError_t launch_heap_selftest(const int *ops, int nops, const uint64_t *keys, uint64_t *storage,
+                                int cap, uint64_t *out, int *status, hipStream_t st);
+
+}  // namespace ua

@@ -1,0 +1,752 @@
+// runtime.cpp -- host side of libuastar: the C ABI of include/uastar.h.
+//
+// A ua_ctx owns one HIP stream and every device allocation. The plain solve
+// (ua_grid_solve / ua_puzzle_solve) runs one query on a cooperative launch whose
+// waves all work on it; batches and served queries run one query per workgroup.
+// Device state is never reset (uAstar calls cudaDeviceReset in every initialize,
+// pathway/GPU-solver.cu:93-94); buffers grow and are reused.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/uastar.h"
+#include "common.hpp"
+#include "kernels.hpp"
+
+using namespace ua;
+
+namespace {
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t n = 0;
+    hipError_t ensure(size_t bytes) {
+        if (bytes <= n) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+        hipError_t e = hipMalloc(&p, bytes ? bytes : 1);
+        if (e == hipSuccess) n = bytes;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    template <class T>
+    T *as() const { return static_cast<T *>(p); }
+};
+
+int bits_for(int64_t v) {  // bits to hold 0..v-1 (at least 1)
+    int b = 1;
+    while ((1ll << b) < v) ++b;
+    return b;
+}
+
+double ticks_ms(int64_t t0, int64_t t1) { return (double)(t1 - t0) / 1e5; }  // s_memrealtime: 100 MHz
+
+const double SQRT2 = 1.4142135623730951;
+
+}  // namespace
+
+struct ua_grid {
+    int H, W, wpr;
+    uint64_t *bm;
+};
+
+struct ServeJob;
+void ua_serve_release(ua_ctx *c);  // serve.cpp
+
+struct ua_ctx {
+    int dev = 0;
+    hipStream_t st = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;  // around each search launch on st
+    double last_kernel_ms = 0;
+    ua_config cfg{};
+    std::string err;
+    int num_cu = 0;
+    size_t total_mem = 0;
+    DevBuf pdb[5];
+    DevBuf d_added;
+    // plain solve
+    DevBuf best, vals, heap, seg, tv, qd, qr, pathbuf;
+    QueryDesc last_q{};
+    QueryResult last_r{};
+    bool last_valid = false;
+    // batch
+    DevBuf b_qd, b_qr, b_head;
+    std::vector<DevBuf> b_arenas;        // one per admitted sub-batch, kept for ua_batch_path
+    std::vector<int> b_arena_of;         // arena index of each query, -1 = state released
+    std::vector<QueryDesc> b_desc;
+    std::vector<QueryResult> b_res;
+    std::vector<int64_t> b_order;
+    // serving
+    std::mutex mu;
+    ServeJob *serve = nullptr;
+};
+
+namespace {
+
+int fail(ua_ctx *c, int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code;
+}
+
+#define HIPCHK(c, expr)                                                                          \
+    do {                                                                                         \
+        hipError_t e__ = (expr);                                                                 \
+        if (e__ != hipSuccess)                                                                   \
+            return fail(c, e__ == hipErrorOutOfMemory ? UA_ERR_OOM : UA_ERR_HIP, "%s: %s (%s:%d)", \
+                        #expr, hipGetErrorString(e__), __FILE__, __LINE__);                      \
+    } while (0)
+
+int status_of(const QueryResult &r) {
+    if (r.status == ST_OK) return r.found ? UA_OK : UA_NO_PATH;
+    return r.status;
+}
+
+// ---- per-query planning -------------------------------------------------------
+
+struct Plan {
+    size_t best_bytes = 0, vals_bytes = 0, heap_bytes = 0;
+    size_t total() const { return best_bytes + vals_bytes + heap_bytes; }
+};
+
+int grid_fill_desc(ua_ctx *c, const ua_grid *g, int sx, int sy, int ex, int ey, int heaps, int heap_scale,
+                   QueryDesc &q, Plan &p) {
+    if (!g || !g->bm) return fail(c, UA_ERR_INVALID, "null grid");
+    if (sx < 0 || sx >= g->H || sy < 0 || sy >= g->W || ex < 0 || ex >= g->H || ey < 0 || ey >= g->W)
+        return fail(c, UA_ERR_INVALID, "start/goal out of range");
+    std::memset(&q, 0, sizeof q);
+    q.kind = KIND_GRID;
+    q.H = g->H;
+    q.W = g->W;
+    q.sx = sx;
+    q.sy = sy;
+    q.ex = ex;
+    q.ey = ey;
+    q.wpr = g->wpr;
+    q.ybits = bits_for(g->W);
+    q.cbits = bits_for(g->H) + q.ybits;
+    q.qshift = std::max(0, q.cbits - 10);
+    q.bitmap = g->bm;
+    q.heaps = heaps;
+    q.max_rounds = c->cfg.max_rounds > 0 ? c->cfg.max_rounds : (1 << 26);
+    int64_t cells = (int64_t)g->H * g->W;
+    int64_t items = std::max<int64_t>(65536, cells / 8) * heap_scale;  // open-list capacity
+    int64_t per_heap = (items + 64ll * heaps - 1) / (64ll * heaps);
+    q.heap_cap = (int)std::min<int64_t>(std::max<int64_t>(per_heap + 1, 4), 1 << 30);
+    p = Plan{};
+    p.best_bytes = (size_t)cells * 8;
+    p.heap_bytes = (size_t)heaps * (size_t)(q.heap_cap - 1) * 512;
+    return UA_OK;
+}
+
+bool board_ok(int n, const uint8_t *t) {
+    if (n != 3 && n != 4) return false;
+    std::vector<int> seen(n * n, 0);
+    for (int i = 0; i < n * n; ++i) {
+        if (t[i] >= n * n || seen[t[i]]) return false;
+        seen[t[i]] = 1;
+    }
+    return true;
+}
+
+// parity of the permutation of the n*n squares (blank = n*n) equals the parity
+// of the blank's Manhattan distance to the bottom-right cell
+bool board_solvable(int n, const uint8_t *t) {
+    int nn = n * n;
+    std::vector<int> p(nn);
+    int blank = 0;
+    for (int i = 0; i < nn; ++i) {
+        p[i] = t[i] ? t[i] - 1 : nn - 1;
+        if (!t[i]) blank = i;
+    }
+    std::vector<char> seen(nn, 0);
+    int cycles = 0;
+    for (int i = 0; i < nn; ++i) {
+        if (seen[i]) continue;
+        ++cycles;
+        for (int j = i; !seen[j]; j = p[j]) seen[j] = 1;
+    }
+    int man = (n - 1 - blank / n) + (n - 1 - blank % n);
+    return ((nn - cycles) & 1) == (man & 1);
+}
+
+uint64_t pack_board(int n, const uint8_t *t) {
+    uint64_t s = 0;
+    for (int i = 0; i < n * n; ++i) s |= (uint64_t)t[i] << (4 * i);
+    return s;
+}
+
+int puzzle_fill_desc(ua_ctx *c, int n, const uint8_t *tiles, int heaps, int hash_log2, QueryDesc &q, Plan &p) {
+    std::memset(&q, 0, sizeof q);
+    q.kind = KIND_PUZZLE;
+    q.n = n;
+    q.start_state = pack_board(n, tiles);
+    q.hash_log2 = hash_log2;
+    q.heaps = heaps;
+    q.max_rounds = c->cfg.max_rounds > 0 ? c->cfg.max_rounds : (1 << 26);
+    int64_t items = std::max<int64_t>(65536, (1ll << hash_log2) / 2);
+    int64_t per_heap = (items + 64ll * heaps - 1) / (64ll * heaps);
+    q.heap_cap = (int)std::max<int64_t>(per_heap + 1, 4);
+    p = Plan{};
+    p.best_bytes = (size_t)8 << hash_log2;
+    p.vals_bytes = (size_t)4 << hash_log2;
+    p.heap_bytes = (size_t)heaps * (size_t)(q.heap_cap - 1) * 512;
+    return UA_OK;
+}
+
+uint64_t pdb_bytes(int n) { return n == 4 ? 576576000ull : 362880ull; }
+
+int ensure_pdb(ua_ctx *c, int n) {
+    if (n != 3 && n != 4) return fail(c, UA_ERR_INVALID, "puzzle side must be 3 or 4");
+    if (c->pdb[n].p) return UA_OK;
+    HIPCHK(c, c->pdb[n].ensure(pdb_bytes(n)));
+    HIPCHK(c, c->d_added.ensure(sizeof(unsigned long long)));
+    hipError_t e = build_pdb(n, c->pdb[n].as<uint8_t>(), c->d_added.as<unsigned long long>(), c->st);
+    if (e != hipSuccess) {
+        c->pdb[n].release();
+        return fail(c, UA_ERR_HIP, "PDB build: %s", hipGetErrorString(e));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return UA_OK;
+}
+
+void fill_result(const QueryDesc &q, const QueryResult &r, ua_result *o) {
+    std::memset(o, 0, sizeof *o);
+    o->status = status_of(r);
+    o->found = r.found && r.status == ST_OK;
+    if (o->found) {
+        if (q.kind == KIND_GRID) {
+            int64_t a, b;
+            key_to_pair(r.goal, &a, &b);
+            o->orth = a;
+            o->diag = b;
+            o->cost = (double)a + (double)b * SQRT2;
+        } else {
+            o->steps = (int64_t)(r.goal >> 32);
+            o->cost = (double)o->steps;
+        }
+    }
+    o->rounds = r.rounds;
+    o->pops = r.pops;
+    o->expanded = r.expanded;
+    o->generated = r.generated;
+    o->inserted = r.inserted;
+    o->device_ms = ticks_ms(r.t_start, r.t_end);
+}
+
+// ---- plain solve (whole GPU, cooperative launch) ----------------------------------
+
+int plain_heaps(ua_ctx *c) {
+    int per_cu = 0;
+    if (wide_max_blocks(&per_cu) != hipSuccess || per_cu < 1) per_cu = 1;
+    int want = c->cfg.plain_heaps > 0 ? c->cfg.plain_heaps : 4 * c->num_cu;
+    int blocks = std::max(1, (want + 3) / 4);
+    blocks = std::min(blocks, per_cu * c->num_cu);
+    return blocks * 4;
+}
+
+int run_wide(ua_ctx *c, QueryDesc &q, const Plan &p) {
+    int heaps = q.heaps;
+    int blocks = heaps / 4;
+    int segcap = heaps * 64 * (q.kind == KIND_GRID ? 8 : 4);
+    HIPCHK(c, c->best.ensure(p.best_bytes));
+    if (p.vals_bytes) HIPCHK(c, c->vals.ensure(p.vals_bytes));
+    HIPCHK(c, c->heap.ensure(p.heap_bytes));
+    HIPCHK(c, c->seg.ensure((size_t)2 * segcap * 8));
+    HIPCHK(c, c->tv.ensure(sizeof(TeamVars)));
+    HIPCHK(c, c->qd.ensure(sizeof(QueryDesc)));
+    HIPCHK(c, c->qr.ensure(sizeof(QueryResult)));
+    q.best = c->best.as<uint64_t>();
+    q.vals = c->vals.as<uint32_t>();
+    q.heap = c->heap.as<uint64_t>();
+    if (q.kind == KIND_GRID) {
+        HIPCHK(c, hipMemsetAsync(q.best, 0xFF, p.best_bytes, c->st));
+    } else {
+        HIPCHK(c, hipMemsetAsync(q.best, 0, p.best_bytes, c->st));
+        HIPCHK(c, hipMemsetAsync(q.vals, 0xFF, p.vals_bytes, c->st));
+    }
+    HIPCHK(c, hipMemsetAsync(c->tv.p, 0, sizeof(TeamVars), c->st));
+    HIPCHK(c, hipMemcpyAsync(c->qd.p, &q, sizeof q, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipEventRecord(c->ev0, c->st));
+    HIPCHK(c, launch_wide(c->qd.as<QueryDesc>(), c->qr.as<QueryResult>(), c->tv.as<TeamVars>(),
+                          c->seg.as<uint64_t>(), segcap, c->pdb[3].as<uint8_t>(), c->pdb[4].as<uint8_t>(),
+                          blocks, c->st));
+    HIPCHK(c, hipEventRecord(c->ev1, c->st));
+    QueryResult r{};
+    HIPCHK(c, hipMemcpyAsync(&r, c->qr.p, sizeof r, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    float kms = 0;
+    HIPCHK(c, hipEventElapsedTime(&kms, c->ev0, c->ev1));
+    c->last_kernel_ms = kms;
+    c->last_q = q;
+    c->last_r = r;
+    c->last_valid = true;
+    return UA_OK;
+}
+
+}  // namespace
+
+// =================================================================== C ABI
+
+extern "C" int ua_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+extern "C" int ua_create(int ordinal, const ua_config *cfg, ua_ctx **out) {
+    if (!out) return UA_ERR_INVALID;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return UA_ERR_NO_DEVICE;
+    if (ordinal < 0 || ordinal >= n) return UA_ERR_INVALID;
+    ua_ctx *c = new ua_ctx();
+    c->dev = ordinal;
+    if (cfg) c->cfg = *cfg;
+    if (hipSetDevice(ordinal) != hipSuccess || hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        delete c;
+        return UA_ERR_HIP;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, ordinal) == hipSuccess) {
+        c->num_cu = prop.multiProcessorCount;
+        c->total_mem = prop.totalGlobalMem;
+    }
+    if (c->num_cu <= 0) c->num_cu = 1;
+    *out = c;
+    return UA_OK;
+}
+
+extern "C" void ua_destroy(ua_ctx *c) {
+    if (!c) return;
+    (void)hipSetDevice(c->dev);
+    (void)hipStreamSynchronize(c->st);
+    ua_serve_release(c);
+    for (auto &b : c->pdb) b.release();
+    for (DevBuf *b : {&c->d_added, &c->best, &c->vals, &c->heap, &c->seg, &c->tv, &c->qd, &c->qr,
+                      &c->pathbuf, &c->b_qd, &c->b_qr, &c->b_head})
+        b->release();
+    for (auto &b : c->b_arenas) b.release();
+    (void)hipEventDestroy(c->ev0);
+    (void)hipEventDestroy(c->ev1);
+    (void)hipStreamDestroy(c->st);
+    delete c;
+}
+
+extern "C" const char *ua_last_error(ua_ctx *c) { return c ? c->err.c_str() : "no context"; }
+
+extern "C" int ua_grid_create(ua_ctx *c, int H, int W, const uint64_t *host_bitmap, ua_grid **out) {
+    if (!c || !out || H <= 0 || W <= 0) return fail(c, UA_ERR_INVALID, "bad grid size");
+    if ((int64_t)H * W > (1ll << 31)) return fail(c, UA_ERR_INVALID, "grid larger than 2^31 cells");
+    ua_grid *g = new ua_grid{H, W, (W + 63) / 64, nullptr};
+    size_t bytes = (size_t)H * g->wpr * 8;
+    if (hipMalloc(&g->bm, bytes) != hipSuccess) {
+        delete g;
+        return fail(c, UA_ERR_OOM, "grid bitmap allocation (%zu bytes)", bytes);
+    }
+    if (host_bitmap) {
+        hipError_t e = hipMemcpyAsync(g->bm, host_bitmap, bytes, hipMemcpyHostToDevice, c->st);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+        if (e != hipSuccess) {
+            (void)hipFree(g->bm);
+            delete g;
+            return fail(c, UA_ERR_HIP, "bitmap upload: %s", hipGetErrorString(e));
+        }
+    }
+    *out = g;
+    return UA_OK;
+}
+
+extern "C" int ua_grid_from_bytes(ua_ctx *c, int H, int W, const uint8_t *bytes, ua_grid **out) {
+    if (!bytes) return fail(c, UA_ERR_INVALID, "null bytes");
+    int rc = ua_grid_create(c, H, W, nullptr, out);
+    if (rc) return rc;
+    DevBuf tmp;
+    HIPCHK(c, tmp.ensure((size_t)H * W));
+    HIPCHK(c, hipMemcpyAsync(tmp.p, bytes, (size_t)H * W, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, launch_grid_pack(tmp.as<uint8_t>(), (*out)->bm, H, W, (*out)->wpr, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    tmp.release();
+    return UA_OK;
+}
+
+extern "C" int ua_grid_generate(ua_ctx *c, int H, int W, uint64_t seed, double block_rate, int sx, int sy,
+                                int ex, int ey, ua_grid **out) {
+    if (!(block_rate >= 0.0 && block_rate <= 1.0)) return fail(c, UA_ERR_INVALID, "block_rate outside [0,1]");
+    int rc = ua_grid_create(c, H, W, nullptr, out);
+    if (rc) return rc;
+    uint64_t thr = (uint64_t)std::floor(block_rate * 4294967296.0);
+    HIPCHK(c, launch_grid_gen((*out)->bm, H, W, (*out)->wpr, seed, thr, sx, sy, ex, ey, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return UA_OK;
+}
+
+extern "C" int ua_grid_download(ua_ctx *c, const ua_grid *g, uint64_t *host) {
+    if (!c || !g || !host) return fail(c, UA_ERR_INVALID, "null argument");
+    HIPCHK(c, hipMemcpyAsync(host, g->bm, (size_t)g->H * g->wpr * 8, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return UA_OK;
+}
+
+extern "C" void ua_grid_free(ua_ctx *c, ua_grid *g) {
+    if (!g) return;
+    if (c) (void)hipStreamSynchronize(c->st);
+    if (g->bm) (void)hipFree(g->bm);
+    delete g;
+}
+
+extern "C" int ua_grid_solve(ua_ctx *c, const ua_grid *g, int sx, int sy, int ex, int ey, ua_result *res) {
+    if (!c || !res) return fail(c, UA_ERR_INVALID, "null argument");
+    c->last_valid = false;
+    int heaps = plain_heaps(c);
+    for (int scale = 1; scale <= 64; scale *= 4) {
+        QueryDesc q;
+        Plan p;
+        int rc = grid_fill_desc(c, g, sx, sy, ex, ey, heaps, scale, q, p);
+        if (rc) return rc;
+        rc = run_wide(c, q, p);
+        if (rc) return rc;
+        if (c->last_r.status != ST_CAPACITY) break;
+    }
+    fill_result(c->last_q, c->last_r, res);
+    res->kernel_ms = c->last_kernel_ms;
+    if (res->status < 0) fail(c, res->status, "grid solve failed with status %d", res->status);
+    return res->status;
+}
+
+extern "C" int ua_grid_path(ua_ctx *c, int32_t *ids, int64_t cap, int64_t *len) {
+    if (!c || !len) return fail(c, UA_ERR_INVALID, "null argument");
+    if (!c->last_valid || c->last_q.kind != KIND_GRID || !c->last_r.found)
+        return fail(c, UA_ERR_INVALID, "no solved grid query");
+    const QueryDesc &q = c->last_q;
+    int64_t maxlen = (int64_t)q.H * q.W;
+    size_t len_off = ((size_t)maxlen * 4 + 255) & ~(size_t)255;
+    HIPCHK(c, c->pathbuf.ensure(len_off + 64));
+    int64_t *d_len = reinterpret_cast<int64_t *>(c->pathbuf.as<char>() + len_off);
+    HIPCHK(c, launch_grid_path(q.best, q.H, q.W, q.sx, q.sy, q.ex, q.ey, c->pathbuf.as<int32_t>(), maxlen,
+                               d_len, c->st));
+    int64_t n = 0;
+    HIPCHK(c, hipMemcpyAsync(&n, d_len, 8, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    if (n < 0) return fail(c, UA_ERR_HIP, "broken parent chain");
+    *len = n;
+    if (!ids) return UA_OK;  // size query
+    if (n > cap) return fail(c, UA_ERR_INVALID, "path needs %lld entries", (long long)n);
+    std::vector<int32_t> tmp(n);
+    HIPCHK(c, hipMemcpy(tmp.data(), c->pathbuf.p, (size_t)n * 4, hipMemcpyDeviceToHost));
+    std::reverse_copy(tmp.begin(), tmp.end(), ids);
+    return UA_OK;
+}
+
+extern "C" int ua_puzzle_build_pdb(ua_ctx *c, int n) {
+    if (!c) return UA_ERR_INVALID;
+    return ensure_pdb(c, n);
+}
+
+extern "C" int ua_puzzle_download_pdb(ua_ctx *c, int n, uint8_t *host, uint64_t size) {
+    if (!c || !host) return fail(c, UA_ERR_INVALID, "null argument");
+    int rc = ensure_pdb(c, n);
+    if (rc) return rc;
+    if (size < pdb_bytes(n)) return fail(c, UA_ERR_INVALID, "buffer too small");
+    HIPCHK(c, hipMemcpy(host, c->pdb[n].p, pdb_bytes(n), hipMemcpyDeviceToHost));
+    return UA_OK;
+}
+
+extern "C" int ua_puzzle_solve(ua_ctx *c, int n, const uint8_t *tiles, ua_result *res) {
+    if (!c || !res || !tiles) return fail(c, UA_ERR_INVALID, "null argument");
+    c->last_valid = false;
+    std::memset(res, 0, sizeof *res);
+    if (!board_ok(n, tiles)) return fail(c, UA_ERR_INVALID, "board is not a permutation of 0..n*n-1");
+    if (!board_solvable(n, tiles)) {
+        res->status = UA_NO_PATH;
+        return UA_NO_PATH;
+    }
+    int rc = ensure_pdb(c, n);
+    if (rc) return rc;
+    int heaps = plain_heaps(c);
+    int hl = c->cfg.hash_log2 > 0 ? c->cfg.hash_log2 : (n == 3 ? 19 : 24);
+    for (int tries = 0; tries < 4; ++tries, hl += 2) {
+        QueryDesc q;
+        Plan p;
+        puzzle_fill_desc(c, n, tiles, heaps, hl, q, p);
+        rc = run_wide(c, q, p);
+        if (rc) return rc;
+        if (c->last_r.status != ST_CAPACITY) break;
+    }
+    fill_result(c->last_q, c->last_r, res);
+    res->kernel_ms = c->last_kernel_ms;
+    if (res->status < 0) fail(c, res->status, "puzzle solve failed with status %d", res->status);
+    return res->status;
+}
+
+extern "C" int ua_puzzle_moves(ua_ctx *c, int32_t *dirs, int64_t cap, int64_t *len) {
+    if (!c || !len) return fail(c, UA_ERR_INVALID, "null argument");
+    if (!c->last_valid || c->last_q.kind != KIND_PUZZLE || !c->last_r.found)
+        return fail(c, UA_ERR_INVALID, "no solved puzzle query");
+    const QueryDesc &q = c->last_q;
+    if (q.start_state == 0 || c->last_r.rounds == 0) {  // start == goal
+        *len = 0;
+        return UA_OK;
+    }
+    const int64_t maxlen = 4096;
+    HIPCHK(c, c->pathbuf.ensure(maxlen * 4 + 64));
+    int64_t *d_len = reinterpret_cast<int64_t *>(c->pathbuf.as<char>() + maxlen * 4);
+    HIPCHK(c, launch_puzzle_moves(q.n, q.best, q.vals, q.hash_log2, c->last_r.goal & 0xffffffffull,
+                                  c->pathbuf.as<int32_t>(), maxlen, d_len, c->st));
+    int64_t n = 0;
+    HIPCHK(c, hipMemcpyAsync(&n, d_len, 8, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    if (n < 0) return fail(c, UA_ERR_HIP, "broken parent chain");
+    *len = n;
+    if (!dirs) return UA_OK;  // size query
+    if (n > cap) return fail(c, UA_ERR_INVALID, "moves need %lld entries", (long long)n);
+    std::vector<int32_t> tmp(n);
+    HIPCHK(c, hipMemcpy(tmp.data(), c->pathbuf.p, (size_t)n * 4, hipMemcpyDeviceToHost));
+    std::reverse_copy(tmp.begin(), tmp.end(), dirs);
+    return UA_OK;
+}
+
+// ------------------------------------------------------------------ batch
+
+namespace {
+
+int batch_waves(ua_ctx *c) {
+    int w = c->cfg.team_waves > 0 ? c->cfg.team_waves : 4;
+    return std::min(8, std::max(1, w));
+}
+
+// Runs queries [descs] (already planned, arena offsets assigned) in one persistent launch.
+int run_batch(ua_ctx *c, std::vector<QueryDesc> &descs, std::vector<QueryResult> &res, int waves) {
+    int nq = (int)descs.size();
+    res.assign(nq, QueryResult{});
+    if (!nq) return UA_OK;
+    int segcap = waves * 64 * 8;
+    int per_cu = 0;
+    HIPCHK(c, batch_max_blocks(waves, segcap, &per_cu));
+    int blocks = std::min(nq, c->num_cu * std::max(1, per_cu));
+    HIPCHK(c, c->b_qd.ensure(sizeof(QueryDesc) * nq));
+    HIPCHK(c, c->b_qr.ensure(sizeof(QueryResult) * nq));
+    HIPCHK(c, c->b_head.ensure(64));
+    HIPCHK(c, hipMemcpyAsync(c->b_qd.p, descs.data(), sizeof(QueryDesc) * nq, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemsetAsync(c->b_head.p, 0, 64, c->st));
+    HIPCHK(c, hipEventRecord(c->ev0, c->st));
+    HIPCHK(c, launch_batch(c->b_qd.as<QueryDesc>(), c->b_qr.as<QueryResult>(), nq, c->b_head.as<uint32_t>(),
+                           segcap, c->pdb[3].as<uint8_t>(), c->pdb[4].as<uint8_t>(), blocks, waves, c->st));
+    HIPCHK(c, hipEventRecord(c->ev1, c->st));
+    HIPCHK(c, hipMemcpyAsync(res.data(), c->b_qr.p, sizeof(QueryResult) * nq, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    float kms = 0;
+    HIPCHK(c, hipEventElapsedTime(&kms, c->ev0, c->ev1));
+    c->last_kernel_ms = kms;
+    return UA_OK;
+}
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+}  // namespace
+
+extern "C" int ua_batch_solve(ua_ctx *c, const ua_query *qs, int64_t nq, ua_result *out) {
+    if (!c || (!qs && nq) || (!out && nq) || nq < 0) return fail(c, UA_ERR_INVALID, "null argument");
+    const int waves = batch_waves(c);
+    size_t free_b = 0, total_b = 0;
+    HIPCHK(c, hipMemGetInfo(&free_b, &total_b));
+    size_t budget = c->cfg.mem_budget ? c->cfg.mem_budget : (size_t)(free_b * 0.85);
+
+    // plan every query; reject bad ones up front
+    std::vector<QueryDesc> descs(nq);
+    std::vector<Plan> plans(nq);
+    std::vector<int32_t> pre(nq, 1);  // 1 = to run, else final status
+    for (int64_t i = 0; i < nq; ++i) {
+        const ua_query &u = qs[i];
+        int rc;
+        if (u.kind == UA_QUERY_GRID) {
+            rc = grid_fill_desc(c, u.grid, u.sx, u.sy, u.ex, u.ey, waves, 1, descs[i], plans[i]);
+        } else if (u.kind == UA_QUERY_PUZZLE) {
+            if (!board_ok(u.n, u.tiles)) rc = fail(c, UA_ERR_INVALID, "query %lld: bad board", (long long)i);
+            else if (!board_solvable(u.n, u.tiles)) rc = UA_NO_PATH;
+            else if ((rc = ensure_pdb(c, u.n)) == UA_OK)
+                rc = puzzle_fill_desc(c, u.n, u.tiles, waves,
+                                      c->cfg.hash_log2 > 0 ? c->cfg.hash_log2 : (u.n == 3 ? 17 : 21), descs[i],
+                                      plans[i]);
+        } else {
+            rc = fail(c, UA_ERR_INVALID, "query %lld: bad kind", (long long)i);
+        }
+        if (rc) pre[i] = rc;
+        else if (plans[i].total() > budget) pre[i] = fail(c, UA_ERR_OOM, "query %lld exceeds the memory budget", (long long)i);
+        else pre[i] = 1;
+    }
+    std::vector<QueryResult> results(nq);
+    for (int64_t i = 0; i < nq; ++i) {
+        std::memset(&out[i], 0, sizeof out[i]);
+        if (pre[i] != 1) out[i].status = pre[i];
+    }
+
+    // largest estimated work first (longest-processing-time order), then admit in
+    // memory-bounded rounds; queries that overflow their heap/hash retry bigger.
+    std::vector<int64_t> todo;
+    for (int64_t i = 0; i < nq; ++i)
+        if (pre[i] == 1) todo.push_back(i);
+    std::stable_sort(todo.begin(), todo.end(), [&](int64_t a, int64_t b) { return plans[a].total() > plans[b].total(); });
+    std::vector<int> scale(nq, 1);
+    c->b_desc.assign(nq, QueryDesc{});
+    c->b_res.assign(nq, QueryResult{});
+    for (auto &b : c->b_arenas) b.release();
+    c->b_arenas.clear();
+    c->b_arena_of.assign(nq, -1);
+    size_t held = 0;
+    for (int attempt = 0; attempt < 4 && !todo.empty(); ++attempt) {
+        std::vector<int64_t> retry;
+        size_t pos = 0;
+        while (pos < todo.size()) {
+            // admit a prefix of todo that fits in the budget
+            auto need = [&](int64_t i) {
+                return align256(plans[i].best_bytes) + align256(plans[i].vals_bytes) + align256(plans[i].heap_bytes);
+            };
+            size_t used = 0, end = pos;
+            while (end < todo.size() && used + need(todo[end]) <= budget) used += need(todo[end++]);
+            if (end == pos) {
+                out[todo[pos]].status = fail(c, UA_ERR_OOM, "query does not fit the budget");
+                pre[todo[pos]] = UA_ERR_OOM;
+                ++pos;
+                continue;
+            }
+            if (held + used > budget) {  // release earlier sub-batches' state (paths no longer available)
+                for (auto &b : c->b_arenas) b.release();
+                for (auto &a : c->b_arena_of) a = -1;
+                held = 0;
+            }
+            c->b_arenas.emplace_back();
+            DevBuf &arena = c->b_arenas.back();
+            HIPCHK(c, arena.ensure(used));
+            held += used;
+            int arena_idx = (int)c->b_arenas.size() - 1;
+            char *base = arena.as<char>();
+            // layout: [grid best tables | puzzle vals] filled with 0xFF, then [puzzle keys] = 0, then heaps
+            std::vector<QueryDesc> sub;
+            size_t off = 0;
+            for (size_t k = pos; k < end; ++k) {
+                int64_t i = todo[k];
+                QueryDesc q = descs[i];
+                if (q.kind == KIND_GRID) {
+                    q.best = reinterpret_cast<uint64_t *>(base + off);
+                    off += align256(plans[i].best_bytes);
+                } else {
+                    q.vals = reinterpret_cast<uint32_t *>(base + off);
+                    off += align256(plans[i].vals_bytes);
+                }
+                sub.push_back(q);
+            }
+            size_t ff_end = off;
+            for (size_t k = pos; k < end; ++k) {
+                int64_t i = todo[k];
+                if (descs[i].kind == KIND_PUZZLE) {
+                    sub[k - pos].best = reinterpret_cast<uint64_t *>(base + off);
+                    off += align256(plans[i].best_bytes);
+                }
+            }
+            size_t zero_end = off;
+            for (size_t k = pos; k < end; ++k) {
+                sub[k - pos].heap = reinterpret_cast<uint64_t *>(base + off);
+                off += align256(plans[todo[k]].heap_bytes);
+            }
+            HIPCHK(c, hipMemsetAsync(base, 0xFF, ff_end, c->st));
+            if (zero_end > ff_end) HIPCHK(c, hipMemsetAsync(base + ff_end, 0, zero_end - ff_end, c->st));
+            std::vector<QueryResult> sr;
+            int rc = run_batch(c, sub, sr, waves);
+            if (rc) return rc;
+            for (size_t k = pos; k < end; ++k) {
+                int64_t i = todo[k];
+                c->b_desc[i] = sub[k - pos];
+                c->b_res[i] = sr[k - pos];
+                c->b_arena_of[i] = arena_idx;
+                if (sr[k - pos].status == ST_CAPACITY && attempt < 3) retry.push_back(i);
+            }
+            pos = end;
+        }
+        for (int64_t i : retry) {  // replan bigger
+            scale[i] *= 4;
+            if (descs[i].kind == KIND_GRID) {
+                const ua_query &u = qs[i];
+                grid_fill_desc(c, u.grid, u.sx, u.sy, u.ex, u.ey, waves, scale[i], descs[i], plans[i]);
+            } else {
+                int hl = descs[i].hash_log2 + 2;
+                puzzle_fill_desc(c, qs[i].n, qs[i].tiles, waves, hl, descs[i], plans[i]);
+            }
+        }
+        todo.swap(retry);
+    }
+    for (int64_t i = 0; i < nq; ++i) {
+        if (pre[i] != 1 || c->b_desc[i].kind == 0) continue;
+        fill_result(c->b_desc[i], c->b_res[i], &out[i]);
+        out[i].latency_ms = out[i].device_ms;
+    }
+    return UA_OK;
+}
+
+extern "C" int ua_batch_path(ua_ctx *c, int64_t i, int32_t *outp, int64_t cap, int64_t *len) {
+    if (!c || !len || i < 0 || i >= (int64_t)c->b_desc.size()) return fail(c, UA_ERR_INVALID, "bad query index");
+    if (c->b_arena_of[i] < 0) return fail(c, UA_ERR_INVALID, "query %lld: search state no longer resident", (long long)i);
+    // temporarily expose query i as "last" and reuse the plain path walkers
+    QueryDesc sq = c->last_q;
+    QueryResult sr = c->last_r;
+    bool sv = c->last_valid;
+    c->last_q = c->b_desc[i];
+    c->last_r = c->b_res[i];
+    c->last_valid = true;
+    int rc = c->last_q.kind == KIND_GRID ? ua_grid_path(c, outp, cap, len) : ua_puzzle_moves(c, outp, cap, len);
+    c->last_q = sq;
+    c->last_r = sr;
+    c->last_valid = sv;
+    return rc;
+}
+
+// ------------------------------------------------------------------ diagnostics
+
+extern "C" int ua_debug_heap_selftest(ua_ctx *c, const int32_t *ops, int nops, const uint64_t *keys, int64_t nkeys,
+                                      int cap_nodes, uint64_t *out_pops, int64_t out_cap, int32_t *final_size) {
+    if (!c || !ops || nops <= 0 || cap_nodes < 2) return fail(c, UA_ERR_INVALID, "bad arguments");
+    int64_t need_keys = 0, npops = 0;
+    for (int i = 0; i < nops; ++i) {
+        if (ops[i] > 64) return fail(c, UA_ERR_INVALID, "insert batches hold at most 64 keys");
+        if (ops[i] >= 0) need_keys += ops[i];
+        else ++npops;
+    }
+    if (need_keys > nkeys || npops * 64 > out_cap) return fail(c, UA_ERR_INVALID, "buffers too small");
+    DevBuf d_ops, d_keys, d_store, d_out, d_st;
+    HIPCHK(c, d_ops.ensure(sizeof(int) * nops));
+    HIPCHK(c, d_keys.ensure(8 * std::max<int64_t>(need_keys, 1)));
+    HIPCHK(c, d_store.ensure((size_t)(cap_nodes - 1) * 512));
+    HIPCHK(c, d_out.ensure(8 * std::max<int64_t>(npops * 64, 1)));
+    HIPCHK(c, d_st.ensure(8));
+    HIPCHK(c, hipMemcpy(d_ops.p, ops, sizeof(int) * nops, hipMemcpyHostToDevice));
+    if (need_keys) HIPCHK(c, hipMemcpy(d_keys.p, keys, 8 * need_keys, hipMemcpyHostToDevice));
+    HIPCHK(c, launch_heap_selftest(d_ops.as<int>(), nops, d_keys.as<uint64_t>(), d_store.as<uint64_t>(), cap_nodes,
+                                   d_out.as<uint64_t>(), d_st.as<int>(), c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    if (npops) HIPCHK(c, hipMemcpy(out_pops, d_out.p, 8 * npops * 64, hipMemcpyDeviceToHost));
+    int st = 0;
+    HIPCHK(c, hipMemcpy(&st, d_st.p, 4, hipMemcpyDeviceToHost));
+    if (final_size) *final_size = st;
+    for (DevBuf *b : {&d_ops, &d_keys, &d_store, &d_out, &d_st}) b->release();
+    return UA_OK;
+}

@@ -1,0 +1,546 @@
+// search.hpp -- the GA* team search (device code shared by every search kernel).
+//
+// One search instance is run by a *team* of wavefronts. Each wavefront owns one
+// batched binary heap (wave_heap.hpp). A round is
+//   phase A  every wave pops its 64 smallest keys, expands them, relaxes the
+//            successors with one 64-bit atomicMin each (grid: dense best-cost
+//            table; puzzle: lock-free open-addressing hash), and appends the
+//            improved successors to the team's insert buffer;
+//   barrier  (the last arriver resets the next round's slot)
+//   phase B  every wave reads the round's minimum popped key (LB) and best goal;
+//            the team stops when goal <= LB; otherwise each wave takes a strided
+//            share of the insert buffer, sorts it and inserts it into its heap.
+// This restates the round of uAstar's kExtractExpand / kAssign / kDeduplicate /
+// kHeapInsert (pathway/GPU-kernel.cuh:169-544; puzzle/GPU-kernel.cuh:215-460) and the
+// host loop's termination test (puzzle/GPU-solver.cuh:243-251), but with the test
+// on the device every round and with duplicate detection done by atomics instead
+// of a sort: an improvement is pushed exactly when the atomicMin lowered the cost.
+//
+// Teams: WgTeam = the waves of one workgroup (batched / served queries, one
+// query per workgroup, pulled from a work queue); GridTeam = every wave of a
+// cooperative launch (the plain one-query-per-launch solve).
+#pragma once
+#ifndef UA_EMU
+#include <hip/hip_runtime.h>
+#endif
+
+#include "common.hpp"
+#include "wave_heap.hpp"
+
+namespace ua {
+
+// ------------------------------------------------------------------ teams
+// A team owns: the two round slots, the two halves of the insert buffer (segcap
+// keys each) and four statistics counters. SLOT_SCOPE is the scope of the
+// round-state exchange between the team's waves; DATA_SCOPE the scope of the
+// best-cost table / hash table that several teams' kernels may touch.
+struct TeamMem {
+    RoundSlot *slot;   // [2]
+    uint64_t *seg;     // [2][segcap]
+    int segcap;
+    uint64_t *stats;   // [4]: pops, expanded, generated, inserted
+};
+
+// The waves of one workgroup. Round state and insert buffer live in LDS, so the
+// exchange between waves is LDS traffic ordered by the workgroup barrier.
+struct WgTeam {
+    static constexpr int SLOT_SCOPE = __HIP_MEMORY_SCOPE_WORKGROUP;
+    static constexpr int DATA_SCOPE = __HIP_MEMORY_SCOPE_AGENT;
+    int tw, W;
+    TeamMem m;
+    __device__ explicit WgTeam(const TeamMem &mem) : tw(threadIdx.x >> 6), W(blockDim.x >> 6), m(mem) {}
+    __device__ bool leader() const { return threadIdx.x == 0; }
+    template <class Hook>
+    __device__ bool sync(Hook hook) const {
+        __syncthreads();
+        if (threadIdx.x == 0) hook();
+        __syncthreads();
+        return true;
+    }
+};
+
+// Every wave of a cooperative launch. Round state lives in global memory and
+// crosses a generation barrier with agent-scope release/acquire fences.
+struct GridTeam {
+    static constexpr int SLOT_SCOPE = __HIP_MEMORY_SCOPE_AGENT;
+    static constexpr int DATA_SCOPE = __HIP_MEMORY_SCOPE_AGENT;
+    static constexpr uint32_t SPIN_LIMIT = 1u << 22;  // a few seconds; then the team aborts
+    int tw, W;
+    TeamMem m;
+    TeamVars *tv;
+    __device__ GridTeam(const TeamMem &mem, TeamVars *t)
+        : tw((int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6))),
+          W((int)(gridDim.x * (blockDim.x >> 6))), m(mem), tv(t) {}
+    __device__ bool leader() const { return blockIdx.x == 0 && threadIdx.x == 0; }
+    // Generation barrier. Every block's thread 0 arrives once; the last one runs the
+    // hook, resets the counter and bumps the generation. A bounded spin turns a lost
+    // block into an abort that every block observes, never a hang.
+    template <class Hook>
+    __device__ bool sync(Hook hook) const {
+        constexpr int SC = SLOT_SCOPE;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __syncthreads();
+        __shared__ uint32_t s_ok;
+        if (threadIdx.x == 0) {
+            uint32_t ok = 1;
+            uint32_t g = __hip_atomic_load(&tv->bar_gen, __ATOMIC_ACQUIRE, SC);
+            uint32_t a = __hip_atomic_fetch_add(&tv->bar_count, 1u, __ATOMIC_ACQ_REL, SC);
+            if (a == gridDim.x - 1) {
+                hook();
+                __hip_atomic_store(&tv->bar_count, 0u, __ATOMIC_RELAXED, SC);
+                __hip_atomic_fetch_add(&tv->bar_gen, 1u, __ATOMIC_RELEASE, SC);
+            } else {
+                uint32_t spins = 0;
+                while (__hip_atomic_load(&tv->bar_gen, __ATOMIC_ACQUIRE, SC) == g) {
+                    __builtin_amdgcn_s_sleep(1);
+                    if (++spins > SPIN_LIMIT || __hip_atomic_load(&tv->abort, __ATOMIC_RELAXED, SC)) {
+                        __hip_atomic_store(&tv->abort, 1u, __ATOMIC_RELAXED, SC);
+                        ok = 0;
+                        break;
+                    }
+                }
+            }
+            if (__hip_atomic_load(&tv->abort, __ATOMIC_RELAXED, SC)) ok = 0;
+            s_ok = ok;
+        }
+        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        return s_ok != 0;
+    }
+};
+
+template <int SC>
+__device__ __forceinline__ uint64_t aload64(const uint64_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, SC);
+}
+template <int SC>
+__device__ __forceinline__ uint32_t aload32(const uint32_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, SC);
+}
+template <int SC>
+__device__ __forceinline__ uint64_t amin64(uint64_t *p, uint64_t v) {
+    return __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, SC);
+}
+template <int SC>
+__device__ __forceinline__ uint32_t amin32(uint32_t *p, uint32_t v) {
+    return __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, SC);
+}
+
+// ------------------------------------------------------------------ grid problem
+// Item key: (fq << cbits) | (x << ybits | y), fq = f_key >> qshift.
+// Best word per cell: (g_key << 4) | dir, dir = move into the cell (8 = start).
+struct GridProb {
+    static constexpr int MAXS = 8;
+    QueryDesc q;
+    uint64_t cmask, ymask;
+    uint64_t goal_id;
+    __device__ explicit GridProb(const QueryDesc &qd) : q(qd) {
+        cmask = (1ull << q.cbits) - 1;
+        ymask = (1ull << q.ybits) - 1;
+        goal_id = (uint64_t)q.ex * (uint64_t)q.W + (uint64_t)q.ey;
+    }
+    __device__ bool open(int x, int y) const {
+        return (q.bitmap[(size_t)x * q.wpr + (y >> 6)] >> (y & 63)) & 1ull;
+    }
+    __device__ uint64_t item(uint64_t fkey, int x, int y) const {
+        return ((fkey >> q.qshift) << q.cbits) | ((uint64_t)x << q.ybits) | (uint64_t)y;
+    }
+    __device__ bool trivial() const { return q.sx == q.ex && q.sy == q.ey; }
+    // Called by one lane: writes the start record and returns its item key.
+    template <int SC>
+    __device__ uint64_t seed() const {
+        uint64_t id = (uint64_t)q.sx * q.W + q.sy;
+        __hip_atomic_store(&q.best[id], (uint64_t)8, __ATOMIC_RELAXED, SC);
+        return item(octile_key(q.sx, q.sy, q.ex, q.ey), q.sx, q.sy);
+    }
+    __device__ bool goal_done(uint64_t goal, uint64_t lb) const {
+        return goal != SENT && goal <= ((lb >> q.cbits) << q.qshift);
+    }
+    // Per-lane expansion of one popped item; successors land in s[0..7] (SENT = none).
+    template <int SC>
+    __device__ void expand(uint64_t it, uint64_t (&s)[MAXS], int &cnt, uint64_t &goalmin,
+                           uint32_t &err, int64_t &n_exp, int64_t &n_gen, int64_t &n_ins) const {
+        uint64_t cell = it & cmask;
+        int x = (int)(cell >> q.ybits), y = (int)(cell & ymask);
+        uint64_t id = (uint64_t)x * q.W + y;
+        uint64_t gk = aload64<SC>(&q.best[id]) >> 4;
+        uint64_t fk = gk + octile_key(x, y, q.ex, q.ey);
+        if ((fk >> q.qshift) != (it >> q.cbits)) return;  // stale: a better entry exists
+        ++n_exp;
+        if (!open(x, y)) return;                          // source-cell mask rule
+#pragma unroll
+        for (int d = 0; d < 8; ++d) {
+            int nx = x + kDX[d], ny = y + kDY[d];
+            if (nx < 0 || nx >= q.H || ny < 0 || ny >= q.W) continue;
+            uint64_t nid = (uint64_t)nx * q.W + ny;
+            bool is_goal = nid == goal_id;
+            if (!is_goal && !open(nx, ny)) continue;        // blocked non-goal: a dead end
+            ++n_gen;
+            uint64_t ng = gk + (d < 4 ? KEY_ONE : KEY_SQRT2);
+            uint64_t nf = ng + octile_key(nx, ny, q.ex, q.ey);
+            if (nf >= KEY_LIMIT) {
+                err |= FLAG_RANGE;
+                continue;
+            }
+            uint64_t old = amin64<SC>(&q.best[nid], (ng << 4) | (uint64_t)d);
+            if ((old >> 4) <= ng) continue;
+            ++n_ins;
+            if (is_goal) {
+                goalmin = goalmin < ng ? goalmin : ng;
+            } else {
+                s[d] = item(nf, nx, ny);
+                ++cnt;
+            }
+        }
+    }
+};
+
+// ------------------------------------------------------------------ puzzle problem
+// Board: 4 bits per cell, cell i at bits 4i..4i+3 (PuzzleStorage<4>, puzzle/storage.hpp:86-139;
+// N = 3 uses the same layout with 9 cells). Pattern databases as in
+// puzzle/GPU-kernel.cuh:71-135: rank of the k-tuple of tile cells, one table per partition.
+template <int N>
+struct PSpec;
+template <>
+struct PSpec<3> {
+    static constexpr int NP = 1;
+    static constexpr int K[1] = {8};
+    static constexpr int T[1][8] = {{1, 2, 3, 4, 5, 6, 7, 8}};
+    static constexpr uint32_t M[1][8] = {{40320, 5040, 720, 120, 24, 6, 2, 1}};
+    static constexpr uint64_t OFF[1] = {0};
+};
+template <>
+struct PSpec<4> {
+    static constexpr int NP = 2;
+    static constexpr int K[2] = {8, 7};
+    static constexpr int T[2][8] = {{1, 2, 3, 4, 5, 6, 9, 13}, {7, 8, 10, 11, 12, 14, 15, 0}};
+    static constexpr uint32_t M[2][8] = {{32432400, 2162160, 154440, 11880, 990, 90, 9, 1},
+                                         {3603600, 240240, 17160, 1320, 110, 10, 1, 0}};
+    static constexpr uint64_t OFF[2] = {0, 518918400ull};
+};
+
+__device__ __forceinline__ int nib_pos(uint64_t s, int t) {
+    uint64_t x = s ^ (0x1111111111111111ull * (uint64_t)t);
+    uint64_t z = (x - 0x1111111111111111ull) & ~x & 0x8888888888888888ull;
+    return __builtin_ctzll(z) >> 2;
+}
+__device__ __forceinline__ uint64_t goal_board(int n) {
+    uint64_t g = 0;
+    for (int i = 0; i < n * n - 1; ++i) g |= (uint64_t)(i + 1) << (4 * i);
+    return g;
+}
+
+template <int N>
+__device__ __forceinline__ uint32_t pattern_rank(uint64_t s, int p) {
+    using S = PSpec<N>;
+    uint32_t mask = 0, code = 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        if (j < S::K[p]) {
+            int pos = nib_pos(s, S::T[p][j]);
+            uint32_t v = (uint32_t)pos - __builtin_popcount(mask & ((1u << pos) - 1u));
+            code += v * S::M[p][j];
+            mask |= 1u << pos;
+        }
+    }
+    return code;
+}
+
+template <int N>
+__device__ __forceinline__ int puzzle_h(uint64_t s, const uint8_t *pdb) {
+    using S = PSpec<N>;
+    int h = 0;
+#pragma unroll
+    for (int p = 0; p < S::NP; ++p) h += pdb[S::OFF[p] + pattern_rank<N>(s, p)];
+    return h;
+}
+
+// Item key: (f << 56) | ((255 - g) << 48) | slot  (ties: larger g first).
+// Hash: keys[slot] = board (0 = empty), vals[slot] = (g << 4) | dir (15 = start).
+template <int N>
+struct PuzzleProb {
+    static constexpr int MAXS = 4;
+    static constexpr int MAXPROBE = 256;
+    QueryDesc q;
+    const uint8_t *pdb;
+    uint64_t hmask;
+    uint64_t goal_b;
+    __device__ PuzzleProb(const QueryDesc &qd, const uint8_t *p) : q(qd), pdb(p) {
+        hmask = (1ull << q.hash_log2) - 1;
+        goal_b = goal_board(N);
+    }
+    __device__ bool trivial() const { return q.start_state == goal_b; }
+    template <int SC>
+    __device__ uint64_t find_or_insert(uint64_t b, uint32_t &err) const {
+        uint64_t slot = mix64(b) & hmask;
+        for (int i = 0; i < MAXPROBE; ++i) {
+            uint64_t k = aload64<SC>(&q.best[slot]);
+            if (k == b) return slot;
+            if (k == 0) {
+                uint64_t exp = 0;
+                if (__hip_atomic_compare_exchange_strong(&q.best[slot], &exp, b, __ATOMIC_RELAXED,
+                                                         __ATOMIC_RELAXED, SC) ||
+                    exp == b)
+                    return slot;
+            }
+            slot = (slot + 1) & hmask;
+        }
+        err |= FLAG_CAPACITY;
+        return SENT;
+    }
+    template <int SC>
+    __device__ uint64_t seed() const {
+        uint32_t err = 0;
+        uint64_t slot = find_or_insert<SC>(q.start_state, err);
+        __hip_atomic_store(&q.vals[slot], (uint32_t)15, __ATOMIC_RELAXED, SC);
+        uint64_t h = (uint64_t)puzzle_h<N>(q.start_state, pdb);
+        return (h << 56) | (255ull << 48) | slot;
+    }
+    __device__ bool goal_done(uint64_t goal, uint64_t lb) const {
+        return goal != SENT && (goal >> 32) <= (lb >> 56);
+    }
+    template <int SC>
+    __device__ void expand(uint64_t it, uint64_t (&s)[MAXS], int &cnt, uint64_t &goalmin,
+                           uint32_t &err, int64_t &n_exp, int64_t &n_gen, int64_t &n_ins) const {
+        uint64_t slot = it & ((1ull << 48) - 1);
+        uint32_t gi = 255u - (uint32_t)((it >> 48) & 255u);
+        uint64_t b = aload64<SC>(&q.best[slot]);
+        uint32_t v = aload32<SC>(&q.vals[slot]);
+        uint32_t gb = v >> 4;
+        if (gb != gi) return;  // stale
+        ++n_exp;
+        int dprev = (int)(v & 15u);
+        int bl = nib_pos(b, 0);
+        int bx = bl / N, by = bl % N;
+#pragma unroll
+        for (int d = 0; d < 4; ++d) {
+            if (dprev < 4 && d == (dprev ^ 1)) continue;  // the move back to the parent
+            int nx = bx + kDX[d], ny = by + kDY[d];
+            if (nx < 0 || nx >= N || ny < 0 || ny >= N) continue;
+            ++n_gen;
+            int nb = nx * N + ny;
+            uint64_t t = (b >> (4 * nb)) & 15ull;
+            uint64_t ns = b + (t << (4 * bl)) - (t << (4 * nb));
+            uint32_t ng = gb + 1;
+            int h = puzzle_h<N>(ns, pdb);
+            if (ng + h > 255u) {
+                err |= FLAG_RANGE;
+                continue;
+            }
+            uint64_t ns_slot = find_or_insert<SC>(ns, err);
+            if (ns_slot == SENT) continue;
+            uint32_t old = amin32<SC>(&q.vals[ns_slot], (ng << 4) | (uint32_t)d);
+            if ((old >> 4) <= ng) continue;
+            ++n_ins;
+            if (ns == goal_b) {
+                uint64_t gk = ((uint64_t)ng << 32) | ns_slot;
+                goalmin = goalmin < gk ? goalmin : gk;
+            } else {
+                s[d] = ((uint64_t)(ng + h) << 56) | ((uint64_t)(255u - ng) << 48) | ns_slot;
+                ++cnt;
+            }
+        }
+    }
+};
+
+// ------------------------------------------------------------------ the team search
+template <class Team, class Prob>
+__device__ void team_solve(const QueryDesc &q, QueryResult *res, const Team &team, const Prob &P) {
+    constexpr int SS = Team::SLOT_SCOPE;
+    constexpr int DS = Team::DATA_SCOPE;
+    const int l = lane_id();
+    const int tw = team.tw, W = team.W;
+    RoundSlot *const slots = team.m.slot;
+    uint64_t *const stats = team.m.stats;
+    const int segcap = team.m.segcap;
+
+    if (P.trivial()) {  // start == goal: cost 0, no search
+        if (team.leader()) {
+            res->status = ST_OK;
+            res->found = 1;
+            res->goal = 0;
+            res->rounds = 0;
+            res->pops = res->expanded = res->generated = res->inserted = 0;
+            res->flags = 0;
+        }
+        return;
+    }
+
+    WaveHeap heap;
+    heap.init(q.heap + (size_t)tw * (size_t)(q.heap_cap - 1) * 64, q.heap_cap);
+    if (team.leader()) {
+        for (int i = 0; i < 2; ++i) {
+            __hip_atomic_store(&slots[i].lb, SENT, __ATOMIC_RELAXED, SS);
+            __hip_atomic_store(&slots[i].goal, SENT, __ATOMIC_RELAXED, SS);
+            __hip_atomic_store(&slots[i].cnt, 0u, __ATOMIC_RELAXED, SS);
+            __hip_atomic_store(&slots[i].err, 0u, __ATOMIC_RELAXED, SS);
+        }
+        for (int i = 0; i < 4; ++i) __hip_atomic_store(&stats[i], (uint64_t)0, __ATOMIC_RELAXED, SS);
+    }
+    if (tw == 0) {
+        uint64_t it = SENT;
+        if (l == 0) it = P.template seed<DS>();
+        heap.part = it;
+        heap.pc = 1;
+    }
+    if (!team.sync([] {})) return;
+
+    int64_t n_pop = 0, n_exp = 0, n_gen = 0, n_ins = 0;
+    uint64_t best_goal = SENT;
+    int32_t status = ST_OK;
+    int found = 0;
+    int r = 0;
+    for (;; ++r) {
+        RoundSlot *cur = &slots[r & 1];
+        uint64_t *cur_seg = team.m.seg + (size_t)(r & 1) * segcap;
+        // ---- phase A: pop, expand, relax, append
+        uint64_t it = heap.pop();
+        uint64_t wmin = readlane64(it, 0);
+        if (l == 0 && wmin != SENT) __hip_atomic_fetch_min(&cur->lb, wmin, __ATOMIC_RELAXED, SS);
+
+        uint64_t s[Prob::MAXS];
+#pragma unroll
+        for (int i = 0; i < Prob::MAXS; ++i) s[i] = SENT;
+        int c = 0;
+        uint64_t goalmin = SENT;
+        uint32_t err = 0;
+        if (it != SENT) {
+            ++n_pop;
+            P.template expand<DS>(it, s, c, goalmin, err, n_exp, n_gen, n_ins);
+        }
+        int total;
+        int off = wave_excl_scan(c, &total);
+        uint32_t base = 0;
+        if (l == 0 && total) base = __hip_atomic_fetch_add(&cur->cnt, (uint32_t)total, __ATOMIC_RELAXED, SS);
+        base = __shfl(base, 0);
+        uint64_t *out = cur_seg + base + off;
+#pragma unroll
+        for (int i = 0; i < Prob::MAXS; ++i)
+            if (s[i] != SENT) *out++ = s[i];
+        goalmin = wave_min64(goalmin);
+        if (l == 0 && goalmin != SENT) __hip_atomic_fetch_min(&cur->goal, goalmin, __ATOMIC_RELAXED, SS);
+        for (int j = 32; j > 0; j >>= 1) err |= __shfl_xor(err, j);
+        if (l == 0 && err) __hip_atomic_fetch_or(&cur->err, err, __ATOMIC_RELAXED, SS);
+
+        RoundSlot *nxt = &slots[(r + 1) & 1];
+        bool ok = team.sync([nxt] {
+            __hip_atomic_store(&nxt->lb, SENT, __ATOMIC_RELAXED, Team::SLOT_SCOPE);
+            __hip_atomic_store(&nxt->goal, SENT, __ATOMIC_RELAXED, Team::SLOT_SCOPE);
+            __hip_atomic_store(&nxt->cnt, 0u, __ATOMIC_RELAXED, Team::SLOT_SCOPE);
+            __hip_atomic_store(&nxt->err, 0u, __ATOMIC_RELAXED, Team::SLOT_SCOPE);
+        });
+        if (!ok) {
+            status = ST_ABORT;
+            break;
+        }
+        // ---- phase B: termination test, then insert this wave's share
+        uint64_t lb = __hip_atomic_load(&cur->lb, __ATOMIC_RELAXED, SS);
+        uint64_t g = __hip_atomic_load(&cur->goal, __ATOMIC_RELAXED, SS);
+        uint32_t cnt = __hip_atomic_load(&cur->cnt, __ATOMIC_RELAXED, SS);
+        uint32_t rerr = __hip_atomic_load(&cur->err, __ATOMIC_RELAXED, SS);
+        best_goal = best_goal < g ? best_goal : g;
+        if (rerr) {
+            status = (rerr & FLAG_CAPACITY) ? ST_CAPACITY : ST_RANGE;
+            break;
+        }
+        if (lb == SENT) {  // every heap was empty: the open list is exhausted
+            found = best_goal != SENT;
+            status = found ? ST_OK : ST_NO_PATH;
+            break;
+        }
+        if (P.goal_done(best_goal, lb)) {
+            found = 1;
+            break;
+        }
+        if (r + 1 >= q.max_rounds) {
+            status = ST_ROUNDS;
+            break;
+        }
+        int rot = (tw + r) % W;
+        int mine = cnt > (uint32_t)rot ? (int)((cnt - (uint32_t)rot + W - 1) / W) : 0;
+        bool fits = true;
+        for (int b0 = 0; b0 < mine; b0 += 64) {
+            int k = b0 + l;
+            uint64_t v = k < mine ? cur_seg[(size_t)k * W + rot] : SENT;
+            int sreal = mine - b0 < 64 ? mine - b0 : 64;
+            v = wave_sort(v);
+            fits &= heap.insert(v, sreal);
+        }
+        if (!fits && l == 0) __hip_atomic_fetch_or(&nxt->err, FLAG_CAPACITY, __ATOMIC_RELAXED, SS);
+    }
+
+    // ---- statistics and result
+    n_pop = wave_sum64(n_pop);
+    n_exp = wave_sum64(n_exp);
+    n_gen = wave_sum64(n_gen);
+    n_ins = wave_sum64(n_ins);
+    if (l == 0) {
+        __hip_atomic_fetch_add(&stats[0], (uint64_t)n_pop, __ATOMIC_RELAXED, SS);
+        __hip_atomic_fetch_add(&stats[1], (uint64_t)n_exp, __ATOMIC_RELAXED, SS);
+        __hip_atomic_fetch_add(&stats[2], (uint64_t)n_gen, __ATOMIC_RELAXED, SS);
+        __hip_atomic_fetch_add(&stats[3], (uint64_t)n_ins, __ATOMIC_RELAXED, SS);
+    }
+    team.sync([] {});
+    if (team.leader()) {
+        res->status = status;
+        res->found = found;
+        res->goal = best_goal;
+        res->rounds = r + 1;
+        res->pops = (int64_t)__hip_atomic_load(&stats[0], __ATOMIC_RELAXED, SS);
+        res->expanded = (int64_t)__hip_atomic_load(&stats[1], __ATOMIC_RELAXED, SS);
+        res->generated = (int64_t)__hip_atomic_load(&stats[2], __ATOMIC_RELAXED, SS);
+        res->inserted = (int64_t)__hip_atomic_load(&stats[3], __ATOMIC_RELAXED, SS) + 1;
+        res->flags = 0;
+    }
+}
+
+template <class Team>
+__device__ void dispatch(const QueryDesc &q, QueryResult *res, const uint8_t *pdb3, const uint8_t *pdb4,
+                         const Team &team) {
+    if (q.kind == KIND_GRID) {
+        GridProb P(q);
+        team_solve(q, res, team, P);
+    } else if (q.n == 4) {
+        PuzzleProb<4> P(q, pdb4);
+        team_solve(q, res, team, P);
+    } else {
+        PuzzleProb<3> P(q, pdb3);
+        team_solve(q, res, team, P);
+    }
+}
+
+__device__ __forceinline__ int64_t now_ticks() { return (int64_t)__builtin_amdgcn_s_memrealtime(); }
+
+// LDS of a workgroup team: 2 round slots, 4 counters, 2 x segcap insert-buffer keys.
+constexpr int WG_LDS_HEAD = 2 * (int)sizeof(RoundSlot) + 4 * 8;
+inline int wg_lds_bytes(int segcap) { return WG_LDS_HEAD + 2 * segcap * 8; }
+
+// Body of the persistent batch kernel for one workgroup: pull a query index,
+// solve it as a team, repeat until the list is exhausted.
+__device__ inline void batch_block(const QueryDesc *qs, QueryResult *rs, int nq, uint32_t *qhead, int segcap,
+                                   const uint8_t *pdb3, const uint8_t *pdb4, uint64_t *lds) {
+    __shared__ int s_q;
+    TeamMem m;
+    m.slot = reinterpret_cast<RoundSlot *>(lds);
+    m.stats = lds + 2 * sizeof(RoundSlot) / 8;
+    m.seg = lds + WG_LDS_HEAD / 8;
+    m.segcap = segcap;
+    WgTeam team(m);
+    for (;;) {
+        if (threadIdx.x == 0) s_q = (int)atomicAdd(qhead, 1u);
+        __syncthreads();
+        int qi = s_q;
+        __syncthreads();
+        if (qi >= nq) break;
+        int64_t t0 = now_ticks();
+        QueryDesc q = qs[qi];
+        dispatch(q, rs + qi, pdb3, pdb4, team);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            rs[qi].t_start = t0;
+            rs[qi].t_end = now_ticks();
+        }
+    }
+}
+
+}  // namespace ua

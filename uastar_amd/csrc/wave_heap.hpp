@@ -11,7 +11,9 @@
 // This replaces uAstar's one-heap-per-thread sift-down/sift-up in global memory
 // (pathway/GPU-kernel.cuh:209-252 and :517-535; puzzle/GPU-kernel.cuh:262-298, :435-453).
 #pragma once
+#ifndef UA_EMU
 #include <hip/hip_runtime.h>
+#endif
 #include "common.hpp"
 
 namespace ua {
