@@ -1,5 +1,5 @@
-// emu_batch.cpp -- runs the device batch loop (ua::batch_block from search.hpp) for one
-// workgroup on host threads. Built by tests/test_emu_search.py into tests/emu/libemu.so.
+// emu_batch.cpp -- runs the device batch loop (ua::group_block from search.hpp) for one
+// block (one group, 4 waves) on host threads. Built by tests/test_emu_search.py.
 #include "hip_emu.hpp"
 
 #include <vector>
@@ -8,9 +8,10 @@
 
 extern "C" int emu_batch(const ua::QueryDesc *qs, ua::QueryResult *rs, int nq, int waves, const uint8_t *pdb3,
                          const uint8_t *pdb4) {
-    if (waves < 1 || waves > 8) return -1;
+    if (waves != 4) return -1;  // k_teams blocks are 4 waves; one block = one group here
     const int segcap = waves * 64 * 8;
-    std::vector<uint64_t> lds((size_t)ua::wg_lds_bytes(segcap) / 8 + 1, 0);
+    std::vector<uint64_t> segs((size_t)2 * segcap, 0);
+    ua::TeamVars tv{};
     uint32_t qhead = 0;
     std::barrier<> bar(waves * 64);
     std::vector<emu::Wave> ws(waves);
@@ -20,7 +21,7 @@ extern "C" int emu_batch(const ua::QueryDesc *qs, ua::QueryResult *rs, int nq, i
     for (int t = 0; t < waves * 64; ++t)
         th.emplace_back([&, t] {
             emu::tid = (unsigned)t;
-            ua::batch_block(qs, rs, nq, &qhead, segcap, pdb3, pdb4, lds.data());
+            ua::group_block(qs, rs, nq, &qhead, &tv, segs.data(), segcap, 1, pdb3, pdb4);
         });
     for (auto &x : th) x.join();
     emu::block = nullptr;

@@ -103,6 +103,9 @@ inline T __shfl_up(T v, unsigned d, int width = 64) {
 inline int __builtin_amdgcn_readlane(int v, int lane) {
     return (int)(uint32_t)emu::exchange((uint64_t)(uint32_t)v, (unsigned)lane);
 }
+inline int __builtin_amdgcn_readfirstlane(int v) {
+    return (int)(uint32_t)emu::exchange((uint64_t)(uint32_t)v, 0u);
+}
 inline uint64_t __builtin_amdgcn_s_memrealtime() {
     return (uint64_t)std::chrono::steady_clock::now().time_since_epoch().count() / 10;
 }

@@ -1,8 +1,8 @@
 """CPU run of the device search code (uastar_amd/csrc/search.hpp) under the SIMT
 emulator in tests/emu/: one workgroup team, host memory, results against the oracle.
 
-This checks the batch kernel's logic (workgroup teams, LDS round exchange,
-per-wave heaps with HBM levels) without a GPU.
+This checks the batch path's logic (one-block groups of k_teams, the group
+barrier and query hand-off, per-wave heaps with HBM levels) without a GPU.
 """
 import ctypes as C
 import os
@@ -119,7 +119,7 @@ def test_emulated_batch_grids(emu, oracle):
 
 def test_emulated_batch_puzzles(emu, oracle):
     pdb3 = oracle.pdb_full(3)
-    b = Batch(waves=2)
+    b = Batch(waves=4)
     exp = []
     for s in range(4):
         t = oracle.puzzle_random(3, 900 + s)

@@ -129,7 +129,7 @@ struct TeamVars {
     uint32_t bar_count;
     uint32_t bar_gen;
     uint32_t abort;
-    uint32_t pad;
+    uint32_t next_q;     // query index handed from the group leader to the group
     uint64_t n_pop, n_exp, n_gen, n_ins;
 };
 

@@ -9,32 +9,13 @@
 
 namespace ua {
 
-// Persistent batch kernel: each workgroup is a team and pulls query indices from
-// a device counter until the list is exhausted (no spin waits).
-__global__ __launch_bounds__(512) void k_batch(const QueryDesc *qs, QueryResult *rs, int nq,
-                                               uint32_t *qhead, int segcap, const uint8_t *pdb3,
-                                               const uint8_t *pdb4) {
-    extern __shared__ uint64_t lds[];
-    batch_block(qs, rs, nq, qhead, segcap, pdb3, pdb4, lds);
-}
-
-// Plain solve: one query, every wave of a cooperative launch is one team.
-__global__ __launch_bounds__(256) void k_wide(const QueryDesc *qs, QueryResult *rs, TeamVars *tv,
-                                              uint64_t *seg, int segcap, const uint8_t *pdb3,
-                                              const uint8_t *pdb4) {
-    TeamMem m;
-    m.slot = tv->slot;
-    m.stats = &tv->n_pop;
-    m.seg = seg;
-    m.segcap = segcap;
-    GridTeam team(m, tv);
-    int64_t t0 = now_ticks();
-    QueryDesc q = qs[0];
-    dispatch(q, rs, pdb3, pdb4, team);
-    if (team.leader()) {
-        rs->t_start = t0;
-        rs->t_end = now_ticks();
-    }
+// The search kernel: a cooperative launch of 4-wave blocks split into groups of
+// `bpg` blocks; each group is a team that solves queries from the shared counter.
+// Plain solve: one group of every block and one query. Batch: one block per group.
+__global__ __launch_bounds__(256) void k_teams(const QueryDesc *qs, QueryResult *rs, int nq, uint32_t *qhead,
+                                               TeamVars *tvs, uint64_t *segs, int segcap, int bpg,
+                                               const uint8_t *pdb3, const uint8_t *pdb4) {
+    group_block(qs, rs, nq, qhead, tvs, segs, segcap, bpg, pdb3, pdb4);
 }
 
 // ------------------------------------------------------------------ path walks
@@ -246,26 +227,16 @@ hipError_t launch_grid_pack(const uint8_t *bytes, uint64_t *bm, int H, int W, in
     return hipGetLastError();
 }
 
-hipError_t launch_batch(const QueryDesc *qs, QueryResult *rs, int nq, uint32_t *qhead, int segcap,
-                        const uint8_t *pdb3, const uint8_t *pdb4, int blocks, int waves, hipStream_t st) {
-    hipLaunchKernelGGL(k_batch, dim3(blocks), dim3(waves * 64), wg_lds_bytes(segcap), st, qs, rs, nq, qhead,
-                       segcap, pdb3, pdb4);
-    return hipGetLastError();
+hipError_t teams_max_blocks(int *per_cu) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_teams, 256, 0);
 }
 
-hipError_t wide_max_blocks(int *per_cu) {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_wide, 256, 0);
-}
-
-hipError_t batch_max_blocks(int waves, int segcap, int *per_cu) {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_batch, waves * 64, wg_lds_bytes(segcap));
-}
-
-hipError_t launch_wide(const QueryDesc *qs, QueryResult *rs, TeamVars *tv, uint64_t *seg, int segcap,
-                       const uint8_t *pdb3, const uint8_t *pdb4, int blocks, hipStream_t st) {
-    void *args[] = {(void *)&qs, (void *)&rs, (void *)&tv, (void *)&seg, (void *)&segcap,
-                    (void *)&pdb3, (void *)&pdb4};
-    return hipLaunchCooperativeKernel((const void *)k_wide, dim3(blocks), dim3(256), args, 0, st);
+hipError_t launch_teams(const QueryDesc *qs, QueryResult *rs, int nq, uint32_t *qhead, TeamVars *tvs,
+                        uint64_t *segs, int segcap, int bpg, const uint8_t *pdb3, const uint8_t *pdb4, int blocks,
+                        hipStream_t st) {
+    void *args[] = {(void *)&qs,   (void *)&rs,     (void *)&nq,  (void *)&qhead, (void *)&tvs,
+                    (void *)&segs, (void *)&segcap, (void *)&bpg, (void *)&pdb3,  (void *)&pdb4};
+    return hipLaunchCooperativeKernel((const void *)k_teams, dim3(blocks), dim3(256), args, 0, st);
 }
 
 hipError_t launch_grid_path(const uint64_t *best, int H, int W, int sx, int sy, int ex, int ey,

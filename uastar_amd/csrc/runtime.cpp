@@ -80,12 +80,12 @@ struct ua_ctx {
     DevBuf pdb[5];
     DevBuf d_added;
     // plain solve
-    DevBuf best, vals, heap, seg, tv, qd, qr, pathbuf;
+    DevBuf best, vals, heap, seg, tv, qd, qr, qh, pathbuf;
     QueryDesc last_q{};
     QueryResult last_r{};
     bool last_valid = false;
     // batch
-    DevBuf b_qd, b_qr, b_head;
+    DevBuf b_qd, b_qr, b_head, b_tv, b_seg;
     std::vector<DevBuf> b_arenas;        // one per admitted sub-batch, kept for ua_batch_path
     std::vector<int> b_arena_of;         // arena index of each query, -1 = state released
     std::vector<QueryDesc> b_desc;
@@ -257,7 +257,7 @@ void fill_result(const QueryDesc &q, const QueryResult &r, ua_result *o) {
 
 int plain_heaps(ua_ctx *c) {
     int per_cu = 0;
-    if (wide_max_blocks(&per_cu) != hipSuccess || per_cu < 1) per_cu = 1;
+    if (teams_max_blocks(&per_cu) != hipSuccess || per_cu < 1) per_cu = 1;
     int want = c->cfg.plain_heaps > 0 ? c->cfg.plain_heaps : 4 * c->num_cu;
     int blocks = std::max(1, (want + 3) / 4);
     blocks = std::min(blocks, per_cu * c->num_cu);
@@ -286,10 +286,12 @@ int run_wide(ua_ctx *c, QueryDesc &q, const Plan &p) {
     }
     HIPCHK(c, hipMemsetAsync(c->tv.p, 0, sizeof(TeamVars), c->st));
     HIPCHK(c, hipMemcpyAsync(c->qd.p, &q, sizeof q, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, c->qh.ensure(64));
+    HIPCHK(c, hipMemsetAsync(c->qh.p, 0, 64, c->st));
     HIPCHK(c, hipEventRecord(c->ev0, c->st));
-    HIPCHK(c, launch_wide(c->qd.as<QueryDesc>(), c->qr.as<QueryResult>(), c->tv.as<TeamVars>(),
-                          c->seg.as<uint64_t>(), segcap, c->pdb[3].as<uint8_t>(), c->pdb[4].as<uint8_t>(),
-                          blocks, c->st));
+    HIPCHK(c, launch_teams(c->qd.as<QueryDesc>(), c->qr.as<QueryResult>(), 1, c->qh.as<uint32_t>(),
+                           c->tv.as<TeamVars>(), c->seg.as<uint64_t>(), segcap, blocks, c->pdb[3].as<uint8_t>(),
+                           c->pdb[4].as<uint8_t>(), blocks, c->st));
     HIPCHK(c, hipEventRecord(c->ev1, c->st));
     QueryResult r{};
     HIPCHK(c, hipMemcpyAsync(&r, c->qr.p, sizeof r, hipMemcpyDeviceToHost, c->st));
@@ -343,8 +345,8 @@ extern "C" void ua_destroy(ua_ctx *c) {
     (void)hipStreamSynchronize(c->st);
     ua_serve_release(c);
     for (auto &b : c->pdb) b.release();
-    for (DevBuf *b : {&c->d_added, &c->best, &c->vals, &c->heap, &c->seg, &c->tv, &c->qd, &c->qr,
-                      &c->pathbuf, &c->b_qd, &c->b_qr, &c->b_head})
+    for (DevBuf *b : {&c->d_added, &c->best, &c->vals, &c->heap, &c->seg, &c->tv, &c->qd, &c->qr, &c->qh,
+                      &c->pathbuf, &c->b_qd, &c->b_qr, &c->b_head, &c->b_tv, &c->b_seg})
         b->release();
     for (auto &b : c->b_arenas) b.release();
     (void)hipEventDestroy(c->ev0);
@@ -530,28 +532,36 @@ extern "C" int ua_puzzle_moves(ua_ctx *c, int32_t *dirs, int64_t cap, int64_t *l
 
 namespace {
 
-int batch_waves(ua_ctx *c) {
+// Waves per batched query: blocks of 4 waves, `team_waves` rounded up to whole blocks.
+int batch_bpg(ua_ctx *c) {
     int w = c->cfg.team_waves > 0 ? c->cfg.team_waves : 4;
-    return std::min(8, std::max(1, w));
+    return std::min(8, std::max(1, (w + 3) / 4));
 }
 
-// Runs queries [descs] (already planned, arena offsets assigned) in one persistent launch.
-int run_batch(ua_ctx *c, std::vector<QueryDesc> &descs, std::vector<QueryResult> &res, int waves) {
+// Runs queries [descs] (already planned, arena offsets assigned) in one cooperative
+// launch of k_teams with one group of `bpg` blocks per concurrently running query.
+int run_batch(ua_ctx *c, std::vector<QueryDesc> &descs, std::vector<QueryResult> &res, int bpg) {
     int nq = (int)descs.size();
     res.assign(nq, QueryResult{});
     if (!nq) return UA_OK;
-    int segcap = waves * 64 * 8;
+    const int waves = 4 * bpg;
+    const int segcap = waves * 64 * 8;
     int per_cu = 0;
-    HIPCHK(c, batch_max_blocks(waves, segcap, &per_cu));
-    int blocks = std::min(nq, c->num_cu * std::max(1, per_cu));
+    HIPCHK(c, teams_max_blocks(&per_cu));
+    int groups = std::max(1, std::min(nq, c->num_cu * std::max(1, per_cu) / bpg));
+    int blocks = groups * bpg;
     HIPCHK(c, c->b_qd.ensure(sizeof(QueryDesc) * nq));
     HIPCHK(c, c->b_qr.ensure(sizeof(QueryResult) * nq));
     HIPCHK(c, c->b_head.ensure(64));
+    HIPCHK(c, c->b_tv.ensure(sizeof(TeamVars) * groups));
+    HIPCHK(c, c->b_seg.ensure((size_t)groups * 2 * segcap * 8));
     HIPCHK(c, hipMemcpyAsync(c->b_qd.p, descs.data(), sizeof(QueryDesc) * nq, hipMemcpyHostToDevice, c->st));
     HIPCHK(c, hipMemsetAsync(c->b_head.p, 0, 64, c->st));
+    HIPCHK(c, hipMemsetAsync(c->b_tv.p, 0, sizeof(TeamVars) * groups, c->st));
     HIPCHK(c, hipEventRecord(c->ev0, c->st));
-    HIPCHK(c, launch_batch(c->b_qd.as<QueryDesc>(), c->b_qr.as<QueryResult>(), nq, c->b_head.as<uint32_t>(),
-                           segcap, c->pdb[3].as<uint8_t>(), c->pdb[4].as<uint8_t>(), blocks, waves, c->st));
+    HIPCHK(c, launch_teams(c->b_qd.as<QueryDesc>(), c->b_qr.as<QueryResult>(), nq, c->b_head.as<uint32_t>(),
+                           c->b_tv.as<TeamVars>(), c->b_seg.as<uint64_t>(), segcap, bpg, c->pdb[3].as<uint8_t>(),
+                           c->pdb[4].as<uint8_t>(), blocks, c->st));
     HIPCHK(c, hipEventRecord(c->ev1, c->st));
     HIPCHK(c, hipMemcpyAsync(res.data(), c->b_qr.p, sizeof(QueryResult) * nq, hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
@@ -567,7 +577,8 @@ size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 extern "C" int ua_batch_solve(ua_ctx *c, const ua_query *qs, int64_t nq, ua_result *out) {
     if (!c || (!qs && nq) || (!out && nq) || nq < 0) return fail(c, UA_ERR_INVALID, "null argument");
-    const int waves = batch_waves(c);
+    const int bpg = batch_bpg(c);
+    const int waves = 4 * bpg;
     size_t free_b = 0, total_b = 0;
     HIPCHK(c, hipMemGetInfo(&free_b, &total_b));
     size_t budget = c->cfg.mem_budget ? c->cfg.mem_budget : (size_t)(free_b * 0.85);
@@ -672,7 +683,7 @@ extern "C" int ua_batch_solve(ua_ctx *c, const ua_query *qs, int64_t nq, ua_resu
             HIPCHK(c, hipMemsetAsync(base, 0xFF, ff_end, c->st));
             if (zero_end > ff_end) HIPCHK(c, hipMemsetAsync(base + ff_end, 0, zero_end - ff_end, c->st));
             std::vector<QueryResult> sr;
-            int rc = run_batch(c, sub, sr, waves);
+            int rc = run_batch(c, sub, sr, bpg);
             if (rc) return rc;
             for (size_t k = pos; k < end; ++k) {
                 int64_t i = todo[k];

@@ -16,9 +16,9 @@
 // on the device every round and with duplicate detection done by atomics instead
 // of a sort: an improvement is pushed exactly when the atomicMin lowered the cost.
 //
-// Teams: WgTeam = the waves of one workgroup (batched / served queries, one
-// query per workgroup, pulled from a work queue); GridTeam = every wave of a
-// cooperative launch (the plain one-query-per-launch solve).
+// Teams: a GroupTeam is a group of blocks of one cooperative launch. The plain
+// one-query-per-launch solve uses one group of every block; batched and served
+// queries use one block (4 waves, 4 heaps) per query, pulled from a work counter.
 #pragma once
 #ifndef UA_EMU
 #include <hip/hip_runtime.h>
@@ -41,40 +41,32 @@ struct TeamMem {
     uint64_t *stats;   // [4]: pops, expanded, generated, inserted
 };
 
-// The waves of one workgroup. Round state and insert buffer live in LDS, so the
-// exchange between waves is LDS traffic ordered by the workgroup barrier.
-struct WgTeam {
-    static constexpr int SLOT_SCOPE = __HIP_MEMORY_SCOPE_WORKGROUP;
-    static constexpr int DATA_SCOPE = __HIP_MEMORY_SCOPE_AGENT;
-    int tw, W;
-    TeamMem m;
-    __device__ explicit WgTeam(const TeamMem &mem) : tw(threadIdx.x >> 6), W(blockDim.x >> 6), m(mem) {}
-    __device__ bool leader() const { return threadIdx.x == 0; }
-    template <class Hook>
-    __device__ bool sync(Hook hook) const {
-        __syncthreads();
-        if (threadIdx.x == 0) hook();
-        __syncthreads();
-        return true;
-    }
-};
-
-// Every wave of a cooperative launch. Round state lives in global memory and
-// crosses a generation barrier with agent-scope release/acquire fences.
-struct GridTeam {
+// A group of `bpg` consecutive blocks of a cooperative launch. Round state and the
+// insert buffer live in global memory and cross a generation barrier over the
+// group's blocks with agent-scope release/acquire fences. The plain solve is one
+// group of every block; a batch is many groups (one block each by default), each
+// pulling queries from a shared counter.
+struct GroupTeam {
     static constexpr int SLOT_SCOPE = __HIP_MEMORY_SCOPE_AGENT;
     static constexpr int DATA_SCOPE = __HIP_MEMORY_SCOPE_AGENT;
-    static constexpr uint32_t SPIN_LIMIT = 1u << 22;  // a few seconds; then the team aborts
-    int tw, W;
+    static constexpr uint32_t SPIN_LIMIT = 1u << 22;  // a few seconds; then the group aborts
+    int tw, W;      // wave index within the group (wave-uniform), waves in the group
+    int gblock;     // block index within the group
+    int bpg;        // blocks per group
     TeamMem m;
     TeamVars *tv;
-    __device__ GridTeam(const TeamMem &mem, TeamVars *t)
-        : tw((int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6))),
-          W((int)(gridDim.x * (blockDim.x >> 6))), m(mem), tv(t) {}
-    __device__ bool leader() const { return blockIdx.x == 0 && threadIdx.x == 0; }
+    __device__ GroupTeam(const TeamMem &mem, TeamVars *t, int blocks_per_group)
+        : m(mem), tv(t) {
+        bpg = blocks_per_group;
+        gblock = (int)(blockIdx.x % (unsigned)bpg);
+        int wpb = (int)(blockDim.x >> 6);
+        tw = __builtin_amdgcn_readfirstlane(gblock * wpb + (int)(threadIdx.x >> 6));
+        W = bpg * wpb;
+    }
+    __device__ bool leader() const { return gblock == 0 && threadIdx.x == 0; }
     // Generation barrier. Every block's thread 0 arrives once; the last one runs the
     // hook, resets the counter and bumps the generation. A bounded spin turns a lost
-    // block into an abort that every block observes, never a hang.
+    // block into an abort that every block of the group observes, never a hang.
     template <class Hook>
     __device__ bool sync(Hook hook) const {
         constexpr int SC = SLOT_SCOPE;
@@ -85,7 +77,7 @@ struct GridTeam {
             uint32_t ok = 1;
             uint32_t g = __hip_atomic_load(&tv->bar_gen, __ATOMIC_ACQUIRE, SC);
             uint32_t a = __hip_atomic_fetch_add(&tv->bar_count, 1u, __ATOMIC_ACQ_REL, SC);
-            if (a == gridDim.x - 1) {
+            if (a == (uint32_t)bpg - 1) {
                 hook();
                 __hip_atomic_store(&tv->bar_count, 0u, __ATOMIC_RELAXED, SC);
                 __hip_atomic_fetch_add(&tv->bar_gen, 1u, __ATOMIC_RELEASE, SC);
@@ -511,32 +503,34 @@ __device__ void dispatch(const QueryDesc &q, QueryResult *res, const uint8_t *pd
 
 __device__ __forceinline__ int64_t now_ticks() { return (int64_t)__builtin_amdgcn_s_memrealtime(); }
 
-// LDS of a workgroup team: 2 round slots, 4 counters, 2 x segcap insert-buffer keys.
-constexpr int WG_LDS_HEAD = 2 * (int)sizeof(RoundSlot) + 4 * 8;
-inline int wg_lds_bytes(int segcap) { return WG_LDS_HEAD + 2 * segcap * 8; }
-
-// Body of the persistent batch kernel for one workgroup: pull a query index,
-// solve it as a team, repeat until the list is exhausted.
-__device__ inline void batch_block(const QueryDesc *qs, QueryResult *rs, int nq, uint32_t *qhead, int segcap,
-                                   const uint8_t *pdb3, const uint8_t *pdb4, uint64_t *lds) {
-    __shared__ int s_q;
+// Body of the grouped cooperative kernel for one block: the group leader takes the
+// next query index from the shared counter and hands it to the group through the
+// group's TeamVars; the group solves it as one team; repeat until the list is
+// exhausted. The index is wave-uniform, so descriptors are read with scalar loads.
+__device__ inline void group_block(const QueryDesc *qs, QueryResult *rs, int nq, uint32_t *qhead,
+                                   TeamVars *tvs, uint64_t *segs, int segcap, int bpg, const uint8_t *pdb3,
+                                   const uint8_t *pdb4) {
+    const int group = (int)(blockIdx.x / (unsigned)bpg);
+    TeamVars *tv = tvs + group;
     TeamMem m;
-    m.slot = reinterpret_cast<RoundSlot *>(lds);
-    m.stats = lds + 2 * sizeof(RoundSlot) / 8;
-    m.seg = lds + WG_LDS_HEAD / 8;
+    m.slot = tv->slot;
+    m.stats = &tv->n_pop;
+    m.seg = segs + (size_t)group * 2 * segcap;
     m.segcap = segcap;
-    WgTeam team(m);
+    GroupTeam team(m, tv, bpg);
     for (;;) {
-        if (threadIdx.x == 0) s_q = (int)atomicAdd(qhead, 1u);
-        __syncthreads();
-        int qi = s_q;
-        __syncthreads();
-        if (qi >= nq) break;
         int64_t t0 = now_ticks();
+        if (!team.sync([tv, qhead] {
+                uint32_t qi = atomicAdd(qhead, 1u);
+                __hip_atomic_store(&tv->next_q, qi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }))
+            break;
+        int qi = __builtin_amdgcn_readfirstlane(
+            (int)__hip_atomic_load(&tv->next_q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        if (qi >= nq) break;
         QueryDesc q = qs[qi];
         dispatch(q, rs + qi, pdb3, pdb4, team);
-        __syncthreads();
-        if (threadIdx.x == 0) {
+        if (team.leader()) {
             rs[qi].t_start = t0;
             rs[qi].t_end = now_ticks();
         }
