@@ -17,6 +17,7 @@ from __future__ import annotations
 import argparse
 import concurrent.futures as cf
 import json
+import math
 import os
 import sys
 import time
@@ -104,14 +105,239 @@ def cpu_baseline_grid(specs, budget_s=20.0, threads=None):
                       f"one query alone on 1 thread: {one * 1e3:.0f} ms (oracle FP32 variant, reference containers)"}
 
 
+# ------------------------------------------------------------------ mixed serving stream
+
+GOLDEN = os.path.join(ROOT, "tests", "golden", "c3_puzzles.json")
+
+
+def mixed_stream(n, seed=5, max_side=8192):
+    """The serving stream: 70% grids (side log-uniform in [512, max_side], obstacles
+    Bernoulli(p), p ~ U[0.2, 0.4], random open start and goal), 30% random solvable
+    15-puzzles (boards of the seeded generator, seeds 3000..3255)."""
+    import uastar_amd as U
+    with open(GOLDEN) as f:
+        boards = json.load(f)
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        if rng.random() < 0.7:
+            side = int(math.floor(math.exp(rng.uniform(math.log(512), math.log(max_side)))))
+            p = float(rng.uniform(0.2, 0.4))
+            gseed = int(rng.integers(1, 1 << 62))
+            pts = []
+            while len(pts) < 2:
+                x, y = int(rng.integers(side)), int(rng.integers(side))
+                if U.cell_open(gseed, p, side, x, y):
+                    pts.append((x, y))
+            out.append(dict(kind="grid", H=side, W=side, seed=gseed, p=p, s=pts[0], g=pts[1]))
+        else:
+            b = boards[int(rng.integers(len(boards)))]
+            out.append(dict(kind="puzzle", board=b["board"], steps=b["steps"], expanded=b["expanded"]))
+    return out
+
+
+def materialize(ctx, specs):
+    for s in specs:
+        if s["kind"] == "grid":
+            s["grid"] = ctx.grid_generate(s["H"], s["W"], s["seed"], s["p"], *s["s"], *s["g"])
+
+
+def answer(r, s):
+    if not r["found"]:
+        return None
+    return (r["orth"], r["diag"]) if s["kind"] == "grid" else r["steps"]
+
+
+def alg_bytes(r, s):
+    return grid_bytes_alg(r) if s["kind"] == "grid" else puzzle_bytes_alg(r)
+
+
+def plain_pass(ctx, specs):
+    """One query per whole-GPU launch, in stream order; returns service times (ms)."""
+    svc, res = [], []
+    for s in specs:
+        t = time.perf_counter()
+        r = ctx.grid_solve(s["grid"], *s["s"], *s["g"]) if s["kind"] == "grid" else ctx.puzzle_solve(4, s["board"])
+        svc.append((time.perf_counter() - t) * 1e3)
+        res.append(r)
+    return svc, res
+
+
+def fifo_latency(arrivals_ms, service_ms):
+    """Latency of a single FIFO server (the plain path) under the given arrivals."""
+    t, lat = 0.0, []
+    for a, d in zip(arrivals_ms, service_ms):
+        t = max(t, a) + d
+        lat.append(t - a)
+    return lat
+
+
+def serve_pass(ctx, specs, arrivals_ms):
+    """Continuous batching: each query is submitted at its arrival time; the loop
+    steps the device and retires queries as they finish."""
+    import uastar_amd as U
+    queries = [ctx.grid_query(s["grid"], *s["s"], *s["g"]) if s["kind"] == "grid"
+               else ctx.puzzle_query(4, s["board"]) for s in specs]
+    n = len(specs)
+    ticket_of, done = {}, {}
+    t0 = time.perf_counter()
+    i = 0
+    last_say = t0
+    while len(done) < n:
+        now = (time.perf_counter() - t0) * 1e3
+        while i < n and arrivals_ms[i] <= now:
+            ticket_of[ctx.serve_submit(queries[i])] = i
+            i += 1
+        active, pending = ctx.serve_step()
+        tnow = (time.perf_counter() - t0) * 1e3
+        for t, r in ctx.serve_poll():
+            done[ticket_of[t]] = (r, tnow)
+        if active == 0 and pending == 0 and i < n:
+            time.sleep(max(0.0, (arrivals_ms[i] - tnow) / 1e3))
+        if time.perf_counter() - last_say > 20:
+            say(f"serve: {len(done)}/{n} done, {active} active, {pending} pending")
+            last_say = time.perf_counter()
+    res = [done[k][0] for k in range(n)]
+    lat = [done[k][1] - arrivals_ms[k] for k in range(n)]
+    span = max(done[k][1] for k in range(n)) - arrivals_ms[0]
+    return res, lat, span
+
+
+def cpu_baseline_mixed(specs, budget_s=25.0, threads=None):
+    """The oracle (restatement of the reference CPU A*; FP32 grid costs, reference
+    containers) on a bounded sample of the stream, one query per host thread."""
+    from oracle import oracle as O
+    threads = threads or min(16, os.cpu_count() or 1)
+    pdb = None
+    # cap the per-query CPU cost so the sample stays bounded
+    cand = [s for s in specs if (s["kind"] == "grid" and s["H"] <= 2048) or
+            (s["kind"] == "puzzle" and s["expanded"] <= 1_500_000)]
+    sample = cand[: 3 * threads]
+
+    def solve(s):
+        nonlocal pdb
+        if s["kind"] == "grid":
+            g = O.grid_bernoulli(s["H"], s["W"], s["seed"], s["p"], *s["s"], *s["g"])
+            t = time.perf_counter()
+            O.grid_astar(g, s["H"], s["W"], *s["s"], *s["g"], "f32", want_path=False)
+        else:
+            t = time.perf_counter()
+            O.puzzle_solve(4, np.array(s["board"], np.uint8), pdb, "astar")
+        return time.perf_counter() - t
+
+    if any(s["kind"] == "puzzle" for s in sample):
+        pdb = O.pdb_full(4, 1, cache_dir=os.path.join(ROOT, "_cache"))
+    t = time.perf_counter()
+    with cf.ThreadPoolExecutor(threads) as ex:
+        times = list(ex.map(solve, sample))
+    wall = time.perf_counter() - t
+    return {"value": len(sample) / wall, "unit": "queries/s", "cores": threads, "kind": "port",
+            "sample": f"{len(sample)} queries of the stream (grids with side <= 2048, puzzles with <= 1.5M "
+                      f"A* expansions) on {threads} threads in {wall:.1f} s; mean single-thread solve "
+                      f"{np.mean(times) * 1e3:.0f} ms (PDB build and input generation excluded)"}
+
+
+def run_serve_mixed(args):
+    import uastar_amd as U
+    ctx = U.Context(0)
+    specs = mixed_stream(args.steps + args.warmup)
+    materialize(ctx, specs)
+    warm, specs = specs[: args.warmup], specs[args.warmup:]
+    ctx.puzzle_build_pdb(4)
+    say(f"warm-up: {len(warm)} plain + served queries")
+    plain_pass(ctx, warm)
+    for s in warm:
+        q = ctx.grid_query(s["grid"], *s["s"], *s["g"]) if s["kind"] == "grid" else ctx.puzzle_query(4, s["board"])
+        ctx.serve_submit(q)
+    ctx.serve_drain()
+    ctx.serve_poll()
+    say(f"plain pass: {len(specs)} queries, one per whole-GPU launch")
+    svc, pres = plain_pass(ctx, specs)
+    plain_qps = len(specs) / (sum(svc) / 1e3)
+    say(f"plain: {plain_qps:.2f} queries/s")
+    rate = args.load * plain_qps  # arrivals per second
+    rng = np.random.default_rng(5)
+    arrivals = np.cumsum(rng.exponential(1e3 / rate, len(specs))).tolist()
+    arrivals = [a - arrivals[0] for a in arrivals]
+    st0 = ctx.serve_stats()
+    say(f"serve pass: Poisson arrivals at {rate:.1f} queries/s ({args.load:g} x plain capacity)")
+    sres, slat, span = serve_pass(ctx, specs, arrivals)
+    st1 = ctx.serve_stats()
+    serve_qps = len(specs) / (span / 1e3)
+    plat = fifo_latency(arrivals, svc)
+    mismatch = sum(answer(a, s) != answer(b, s) for a, b, s in zip(pres, sres, specs))
+    errors = sum(r["status"] < 0 for r in pres + sres)
+    kms = st1["kernel_ms"] - st0["kernel_ms"]
+    alg = sum(alg_bytes(r, s) for r, s in zip(sres, specs))
+    achieved = alg / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
+    out = {
+        "metric": "queries/s + p50/p99 latency, batched serving vs plain GA*, 1 MI355X, exact",
+        "value": serve_qps,
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": len(specs),
+        "warmup": len(warm),
+        "ms_per_step": span / len(specs),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int64",
+        "data": "synthetic (seeded generators)",
+        "config": {"workload": "serving stream: 70% grids (side log-uniform in [512, 8192], Bernoulli(p) "
+                               "obstacles with p ~ U[0.2,0.4], random open start/goal), 30% random solvable "
+                               "15-puzzles with the 7-8 PDB; Poisson arrivals at load x plain capacity",
+                   "mode": "serve (continuous batching, one 4-wave team per query)", "load": args.load,
+                   "arrival_rate_qps": rate},
+        "p50_ms": pct(slat, 50),
+        "p99_ms": pct(slat, 99),
+        "plain": {"queries_per_s": plain_qps, "p50_ms": pct(plat, 50), "p99_ms": pct(plat, 99),
+                  "service_p50_ms": pct(svc, 50), "service_p99_ms": pct(svc, 99)},
+        "speedup_vs_plain": serve_qps / plain_qps,
+        "exact_mismatches_serve_vs_plain": mismatch,
+        "errors": errors,
+        "serve_launches": st1["launches"] - st0["launches"],
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_teams (serve launches)",
+                     "kernel_ms_total": kms, "alg_bytes_total": alg},
+    }
+    def diag(res, lat=None):
+        rows = []
+        for k, (r, sp) in enumerate(zip(res, specs)):
+            rows.append(dict(kind=sp["kind"], size=sp.get("H", 0), rounds=r["rounds"], expanded=r["expanded"],
+                             dev_ms=r["device_ms"], us_per_round=1e3 * r["device_ms"] / max(1, r["rounds"]),
+                             lat_ms=lat[k] if lat else None))
+        by = {}
+        for kind in ("grid", "puzzle"):
+            rr = [x for x in rows if x["kind"] == kind]
+            if rr:
+                by[kind] = {"n": len(rr), "rounds_p50": pct([x["rounds"] for x in rr], 50),
+                            "rounds_max": max(x["rounds"] for x in rr),
+                            "us_per_round_p50": pct([x["us_per_round"] for x in rr], 50),
+                            "expanded_p50": pct([x["expanded"] for x in rr], 50)}
+        return rows, by
+    srows, out["serve_diag"] = diag(sres, slat)
+    _, out["plain_diag"] = diag(pres)
+    for x in sorted(srows, key=lambda x: -(x["lat_ms"] or 0))[:8]:
+        say(f"slowest served: {x}")
+    if not args.no_cpu:
+        say("CPU baseline (oracle, bounded sample)")
+        out["cpu_baseline"] = cpu_baseline_mixed(specs)
+    for s in specs + warm:
+        if s["kind"] == "grid":
+            s["grid"].free()
+    ctx.close()
+    return out
+
+
 # ------------------------------------------------------------------ main
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=40)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="plain-grid1000")
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--workload", default="serve-mixed", choices=["serve-mixed", "plain-grid1000"])
+    ap.add_argument("--load", type=float, default=20.0, help="arrival rate as a multiple of plain capacity")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -123,6 +349,9 @@ def main():
     if U.device_count() < 1:
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if args.workload == "serve-mixed":
+        print(json.dumps(run_serve_mixed(args)))
+        return
     ctx = U.Context(0)
 
     specs = grid1000_specs(args.steps)

@@ -42,11 +42,13 @@ typedef struct ua_ctx ua_ctx;
 typedef struct ua_grid ua_grid; /* a device-resident grid bitmap */
 
 typedef struct {
-    int32_t team_waves;  /* waves (= heaps) per query in batch/serve; 0 = autotuned */
-    int32_t plain_heaps; /* heaps (= waves) of the plain whole-GPU solve; 0 = one per CU */
-    int32_t max_rounds;  /* 0 = 1<<26 */
-    int32_t hash_log2;   /* puzzle hash table size per query; 0 = autotuned */
-    uint64_t mem_budget; /* device bytes the batch/serve planner may use; 0 = 85% of free */
+    int32_t team_waves;        /* waves (= heaps) per grid query in batch/serve; 0 = 4 */
+    int32_t plain_heaps;       /* heaps (= waves) of the plain whole-GPU solve; 0 = 4 per CU */
+    int32_t max_rounds;        /* 0 = 1<<26 */
+    int32_t hash_log2;         /* puzzle hash table size per query; 0 = autotuned */
+    uint64_t mem_budget;       /* device bytes the batch/serve planner may use; 0 = 80-85% of free */
+    int32_t puzzle_team_waves; /* waves (= heaps) per puzzle query in batch/serve; 0 = 64 */
+    int32_t reserved;
 } ua_config;
 
 typedef struct {
@@ -125,6 +127,8 @@ int ua_serve_step(ua_ctx *ctx, int64_t *n_active, int64_t *n_pending);
 int ua_serve_poll(ua_ctx *ctx, uint64_t *tickets, ua_result *results, int64_t cap, int64_t *n_out);
 /* steps until nothing is pending or active */
 int ua_serve_drain(ua_ctx *ctx);
+/* launches so far, their summed duration (HIP events) and the current round budget */
+int ua_serve_stats(ua_ctx *ctx, int64_t *launches, double *kernel_ms, int32_t *round_budget);
 
 /* ---- diagnostics */
 /* Runs a random insert/pop sequence on one wave's heap; see tests. */

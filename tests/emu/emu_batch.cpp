@@ -12,7 +12,12 @@ extern "C" int emu_batch(const ua::QueryDesc *qs, ua::QueryResult *rs, int nq, i
     const int segcap = waves * 64 * 8;
     std::vector<uint64_t> segs((size_t)2 * segcap, 0);
     ua::TeamVars tv{};
-    uint32_t qhead = 0;
+    uint32_t qhead[2] = {0, 0};
+    ua::LaunchShape shape{};
+    shape.groups[1] = 1;  // one one-block group of class 1 takes every query
+    shape.bpg[0] = shape.bpg[1] = 1;
+    shape.nq[1] = nq;
+    shape.segcap[1] = segcap;
     std::barrier<> bar(waves * 64);
     std::vector<emu::Wave> ws(waves);
     emu::Block blk{&bar, ws.data(), (unsigned)(waves * 64)};
@@ -21,7 +26,7 @@ extern "C" int emu_batch(const ua::QueryDesc *qs, ua::QueryResult *rs, int nq, i
     for (int t = 0; t < waves * 64; ++t)
         th.emplace_back([&, t] {
             emu::tid = (unsigned)t;
-            ua::group_block(qs, rs, nq, &qhead, &tv, segs.data(), segcap, 1, pdb3, pdb4);
+            ua::group_block(qs, rs, shape, qhead, &tv, segs.data(), pdb3, pdb4);
         });
     for (auto &x : th) x.join();
     emu::block = nullptr;

@@ -24,7 +24,9 @@ class QueryDesc(C.Structure):
     _fields_ = [(n, C.c_int32) for n in ("kind", "n", "H", "W", "sx", "sy", "ex", "ey", "wpr", "ybits", "cbits",
                                           "qshift", "heap_cap", "hash_log2", "max_rounds", "heaps")] + [
         ("bitmap", C.c_void_p), ("best", C.c_void_p), ("vals", C.c_void_p), ("heap", C.c_void_p),
-        ("start_state", C.c_uint64)]
+        ("start_state", C.c_uint64), ("resume", C.c_void_p), ("round_budget", C.c_int32), ("pad2", C.c_int32)]
+
+RESUME_WORDS_4 = 8 + 4 * 130
 
 
 class QueryResult(C.Structure):
@@ -128,3 +130,44 @@ def test_emulated_batch_puzzles(emu, oracle):
     res = b.run(emu, pdb3=pdb3)
     for r, e in zip(res, exp):
         assert r.status == 0 and r.found and (r.goal >> 32) == e
+
+
+def test_emulated_suspend_resume(emu, oracle):
+    """Queries run a few rounds per launch, save their state to the resume record and
+    continue in the next launch; the costs equal the oracle's."""
+    rng = np.random.default_rng(9)
+    b = Batch(waves=4)
+    exp = []
+    for i in range(3):
+        H, W = int(rng.integers(20, 50)), int(rng.integers(20, 50))
+        m = random_grid(rng, H, W, 0.25)
+        m[0, 0] = True
+        g = grid_bytes(m)
+        b.add_grid(g, H, W, 0, 0, H - 1, W - 1, items=4096)
+        exp.append(oracle.grid_astar(g, H, W, 0, 0, H - 1, W - 1, "ref"))
+    pdb3 = oracle.pdb_full(3)
+    t = oracle.puzzle_random(3, 31337)
+    b.add_puzzle(3, t)
+    exp_p = oracle.puzzle_solve(3, t, pdb3, "astar")[0]
+    recs = [np.zeros(RESUME_WORDS_4, np.uint64) for _ in b.descs]
+    for d, r in zip(b.descs, recs):
+        d.resume = r.ctypes.data
+        d.round_budget = 3
+    final = [None] * len(b.descs)
+    launches = 0
+    while any(f is None for f in final):
+        idx = [i for i, f in enumerate(final) if f is None]
+        sub = Batch(waves=4)
+        sub.descs = [b.descs[i] for i in idx]
+        res = sub.run(emu, pdb3=pdb3)
+        launches += 1
+        for i, r in zip(idx, res):
+            if r.status != 2:  # not suspended
+                final[i] = r
+        assert launches < 200
+    assert launches > 2
+    for r, e in zip(final[:3], exp):
+        assert r.status in (0, 1) and bool(r.found) == e["found"]
+        if e["found"]:
+            assert U.key_to_pair(r.goal) == (e["orth"], e["diag"])
+    assert final[3].status == 0 and (final[3].goal >> 32) == exp_p

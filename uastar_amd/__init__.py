@@ -36,7 +36,8 @@ class UAError(RuntimeError):
 
 class Config(C.Structure):
     _fields_ = [("team_waves", C.c_int32), ("plain_heaps", C.c_int32), ("max_rounds", C.c_int32),
-                ("hash_log2", C.c_int32), ("mem_budget", C.c_uint64)]
+                ("hash_log2", C.c_int32), ("mem_budget", C.c_uint64), ("puzzle_team_waves", C.c_int32),
+                ("reserved", C.c_int32)]
 
 
 class Result(C.Structure):
@@ -89,6 +90,7 @@ def lib():
             "ua_serve_step": (C.c_int, [vp, C.POINTER(i64), C.POINTER(i64)]),
             "ua_serve_poll": (C.c_int, [vp, C.POINTER(u64), C.POINTER(Result), i64, C.POINTER(i64)]),
             "ua_serve_drain": (C.c_int, [vp]),
+            "ua_serve_stats": (C.c_int, [vp, C.POINTER(i64), C.POINTER(C.c_double), C.POINTER(i32)]),
             "ua_debug_heap_selftest": (C.c_int, [vp, vp, C.c_int, vp, i64, C.c_int, vp, i64,
                                                  C.POINTER(i32)]),
         }
@@ -171,8 +173,9 @@ class Grid:
 class Context:
     """One device context (ua_ctx)."""
 
-    def __init__(self, ordinal=0, team_waves=0, plain_heaps=0, max_rounds=0, hash_log2=0, mem_budget=0):
-        self.cfg = Config(team_waves, plain_heaps, max_rounds, hash_log2, mem_budget)
+    def __init__(self, ordinal=0, team_waves=0, plain_heaps=0, max_rounds=0, hash_log2=0, mem_budget=0,
+                 puzzle_team_waves=0):
+        self.cfg = Config(team_waves, plain_heaps, max_rounds, hash_log2, mem_budget, puzzle_team_waves, 0)
         h = C.c_void_p()
         rc = lib().ua_create(ordinal, C.byref(self.cfg), C.byref(h))
         if rc:
@@ -298,6 +301,11 @@ class Context:
 
     def serve_drain(self):
         self._check(lib().ua_serve_drain(self.h))
+
+    def serve_stats(self):
+        n, ms, rb = C.c_int64(), C.c_double(), C.c_int32()
+        self._check(lib().ua_serve_stats(self.h, C.byref(n), C.byref(ms), C.byref(rb)))
+        return {"launches": n.value, "kernel_ms": ms.value, "round_budget": rb.value}
 
     # ---- diagnostics
     def heap_selftest(self, ops, keys, cap_nodes):

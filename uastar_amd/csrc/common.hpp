@@ -80,6 +80,7 @@ enum : int32_t {
     ST_ROUNDS = -7,
     ST_NO_PDB = -8,
     ST_ABORT = -9,
+    ST_SUSPENDED = 2,   // round budget of a serving launch used up; state saved for resumption
 };
 // error bits raised by the kernels
 enum : uint32_t { FLAG_CAPACITY = 1, FLAG_RANGE = 2, FLAG_ABORT = 4 };
@@ -101,7 +102,28 @@ struct QueryDesc {
     uint32_t *vals;          // puzzle: hash values (g << 4 | dir)
     uint64_t *heap;          // heap node storage, heaps * (heap_cap-1) * 64 u64
     uint64_t start_state;    // puzzle
+    uint64_t *resume;        // serving: resume record (resume_words(heaps) u64), or null
+    int32_t round_budget;    // serving: rounds per launch before suspending (0 = no limit)
+    int32_t pad2;
 };
+
+// Shape of one k_teams launch: two classes of groups. Class c has groups[c] groups
+// of bpg[c] blocks (class 0 first in block order); its groups pull the queries
+// qs[qoff[c] .. qoff[c] + nq[c]) from counter qhead[c]. A group's insert buffer
+// holds segcap[c] keys per round half.
+struct LaunchShape {
+    int32_t groups[2];
+    int32_t bpg[2];
+    int32_t nq[2];
+    int32_t qoff[2];
+    int32_t segcap[2];
+};
+
+// Resume record of a suspended query: header, then per heap its root batch,
+// partial batch and (n, pc). Written when a serving launch's round budget runs out.
+constexpr int RESUME_HEADER = 8;   // valid, round, best_goal, pops, expanded, generated, inserted, spare
+constexpr int RESUME_PER_HEAP = 130;
+UA_HD inline int64_t resume_words(int heaps) { return RESUME_HEADER + (int64_t)heaps * RESUME_PER_HEAP; }
 
 struct QueryResult {
     int32_t status;

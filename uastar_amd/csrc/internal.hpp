@@ -1,0 +1,119 @@
+// internal.hpp -- host runtime internals shared by runtime.cpp (plain and batch
+// solves) and serve.cpp (the serving engine). Not part of the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/uastar.h"
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace uart {
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t n = 0;
+    hipError_t ensure(size_t bytes) {
+        if (bytes <= n) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+        hipError_t e = hipMalloc(&p, bytes ? bytes : 1);
+        if (e == hipSuccess) n = bytes;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    template <class T>
+    T *as() const { return static_cast<T *>(p); }
+};
+
+inline int bits_for(int64_t v) {  // bits to hold 0..v-1 (at least 1)
+    int b = 1;
+    while ((1ll << b) < v) ++b;
+    return b;
+}
+
+inline double ticks_ms(int64_t t0, int64_t t1) { return (double)(t1 - t0) / 1e5; }  // s_memrealtime: 100 MHz
+
+constexpr double SQRT2 = 1.4142135623730951;
+
+}  // namespace uart
+
+struct ua_grid {
+    int H, W, wpr;
+    uint64_t *bm;
+};
+
+struct ServeEngine;
+void ua_serve_release(ua_ctx *c);  // serve.cpp
+
+struct ua_ctx {
+    int dev = 0;
+    hipStream_t st = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;  // around each search launch on st
+    double last_kernel_ms = 0;
+    ua_config cfg{};
+    std::string err;
+    int num_cu = 0;
+    size_t total_mem = 0;
+    uart::DevBuf pdb[5];
+    uart::DevBuf d_added;
+    // plain solve
+    uart::DevBuf best, vals, heap, seg, tv, qd, qr, qh, pathbuf;
+    ua::QueryDesc last_q{};
+    ua::QueryResult last_r{};
+    bool last_valid = false;
+    // batch
+    uart::DevBuf b_qd, b_qr, b_head, b_tv, b_seg;
+    std::vector<uart::DevBuf> b_arenas;        // one per admitted sub-batch, kept for ua_batch_path
+    std::vector<int> b_arena_of;         // arena index of each query, -1 = state released
+    std::vector<ua::QueryDesc> b_desc;
+    std::vector<ua::QueryResult> b_res;
+    std::vector<int64_t> b_order;
+    // serving
+    std::mutex mu;
+    ServeEngine *serve = nullptr;
+};
+
+
+namespace uart {
+using namespace ua;
+
+int fail(ua_ctx *c, int code, const char *fmt, ...);
+
+#define HIPCHK(c, expr)                                                                          \
+    do {                                                                                         \
+        hipError_t e__ = (expr);                                                                 \
+        if (e__ != hipSuccess)                                                                   \
+            return uart::fail(c, e__ == hipErrorOutOfMemory ? UA_ERR_OOM : UA_ERR_HIP,            \
+                              "%s: %s (%s:%d)", #expr, hipGetErrorString(e__), __FILE__, __LINE__); \
+    } while (0)
+
+struct Plan {
+    size_t best_bytes = 0, vals_bytes = 0, heap_bytes = 0;
+    size_t total() const { return best_bytes + vals_bytes + heap_bytes; }
+};
+
+inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+int status_of(const QueryResult &r);
+int grid_fill_desc(ua_ctx *c, const ua_grid *g, int sx, int sy, int ex, int ey, int heaps, int heap_scale,
+                   QueryDesc &q, Plan &p);
+bool board_ok(int n, const uint8_t *t);
+bool board_solvable(int n, const uint8_t *t);
+uint64_t pack_board(int n, const uint8_t *t);
+int puzzle_fill_desc(ua_ctx *c, int n, const uint8_t *tiles, int heaps, int hash_log2, QueryDesc &q, Plan &p);
+int ensure_pdb(ua_ctx *c, int n);
+void fill_result(const QueryDesc &q, const QueryResult &r, ua_result *o);
+int team_bpg(ua_ctx *c, int kind);
+int run_batch(ua_ctx *c, std::vector<QueryDesc> &descs, std::vector<QueryResult> &res);
+
+}  // namespace uart

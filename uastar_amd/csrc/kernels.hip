@@ -9,13 +9,14 @@
 
 namespace ua {
 
-// The search kernel: a cooperative launch of 4-wave blocks split into groups of
-// `bpg` blocks; each group is a team that solves queries from the shared counter.
-// Plain solve: one group of every block and one query. Batch: one block per group.
-__global__ __launch_bounds__(256) void k_teams(const QueryDesc *qs, QueryResult *rs, int nq, uint32_t *qhead,
-                                               TeamVars *tvs, uint64_t *segs, int segcap, int bpg,
+// The search kernel: a cooperative launch of 4-wave blocks split into groups (see
+// LaunchShape); each group is a team that solves queries of its class from a counter.
+// Plain solve: one group of every block and one query. Batch / serve: wide groups for
+// puzzles, one-block groups for grids.
+__global__ __launch_bounds__(256) void k_teams(const QueryDesc *qs, QueryResult *rs, LaunchShape shape,
+                                               uint32_t *qhead, TeamVars *tvs, uint64_t *segs,
                                                const uint8_t *pdb3, const uint8_t *pdb4) {
-    group_block(qs, rs, nq, qhead, tvs, segs, segcap, bpg, pdb3, pdb4);
+    group_block(qs, rs, shape, qhead, tvs, segs, pdb3, pdb4);
 }
 
 // ------------------------------------------------------------------ path walks
@@ -231,11 +232,12 @@ hipError_t teams_max_blocks(int *per_cu) {
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_teams, 256, 0);
 }
 
-hipError_t launch_teams(const QueryDesc *qs, QueryResult *rs, int nq, uint32_t *qhead, TeamVars *tvs,
-                        uint64_t *segs, int segcap, int bpg, const uint8_t *pdb3, const uint8_t *pdb4, int blocks,
-                        hipStream_t st) {
-    void *args[] = {(void *)&qs,   (void *)&rs,     (void *)&nq,  (void *)&qhead, (void *)&tvs,
-                    (void *)&segs, (void *)&segcap, (void *)&bpg, (void *)&pdb3,  (void *)&pdb4};
+hipError_t launch_teams(const QueryDesc *qs, QueryResult *rs, const LaunchShape &shape, uint32_t *qhead,
+                        TeamVars *tvs, uint64_t *segs, const uint8_t *pdb3, const uint8_t *pdb4, hipStream_t st) {
+    int blocks = shape.groups[0] * shape.bpg[0] + shape.groups[1] * shape.bpg[1];
+    LaunchShape sh = shape;
+    void *args[] = {(void *)&qs, (void *)&rs, (void *)&sh, (void *)&qhead, (void *)&tvs, (void *)&segs,
+                    (void *)&pdb3, (void *)&pdb4};
     return hipLaunchCooperativeKernel((const void *)k_teams, dim3(blocks), dim3(256), args, 0, st);
 }
 

@@ -19,84 +19,12 @@
 #include <string>
 #include <vector>
 
-#include "../../include/uastar.h"
-#include "common.hpp"
-#include "kernels.hpp"
+#include "internal.hpp"
 
 using namespace ua;
+using namespace uart;
 
-namespace {
-
-struct DevBuf {
-    void *p = nullptr;
-    size_t n = 0;
-    hipError_t ensure(size_t bytes) {
-        if (bytes <= n) return hipSuccess;
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-        hipError_t e = hipMalloc(&p, bytes ? bytes : 1);
-        if (e == hipSuccess) n = bytes;
-        return e;
-    }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-    template <class T>
-    T *as() const { return static_cast<T *>(p); }
-};
-
-int bits_for(int64_t v) {  // bits to hold 0..v-1 (at least 1)
-    int b = 1;
-    while ((1ll << b) < v) ++b;
-    return b;
-}
-
-double ticks_ms(int64_t t0, int64_t t1) { return (double)(t1 - t0) / 1e5; }  // s_memrealtime: 100 MHz
-
-const double SQRT2 = 1.4142135623730951;
-
-}  // namespace
-
-struct ua_grid {
-    int H, W, wpr;
-    uint64_t *bm;
-};
-
-struct ServeJob;
-void ua_serve_release(ua_ctx *c);  // serve.cpp
-
-struct ua_ctx {
-    int dev = 0;
-    hipStream_t st = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;  // around each search launch on st
-    double last_kernel_ms = 0;
-    ua_config cfg{};
-    std::string err;
-    int num_cu = 0;
-    size_t total_mem = 0;
-    DevBuf pdb[5];
-    DevBuf d_added;
-    // plain solve
-    DevBuf best, vals, heap, seg, tv, qd, qr, qh, pathbuf;
-    QueryDesc last_q{};
-    QueryResult last_r{};
-    bool last_valid = false;
-    // batch
-    DevBuf b_qd, b_qr, b_head, b_tv, b_seg;
-    std::vector<DevBuf> b_arenas;        // one per admitted sub-batch, kept for ua_batch_path
-    std::vector<int> b_arena_of;         // arena index of each query, -1 = state released
-    std::vector<QueryDesc> b_desc;
-    std::vector<QueryResult> b_res;
-    std::vector<int64_t> b_order;
-    // serving
-    std::mutex mu;
-    ServeJob *serve = nullptr;
-};
-
-namespace {
+namespace uart {
 
 int fail(ua_ctx *c, int code, const char *fmt, ...) {
     char buf[512];
@@ -108,25 +36,12 @@ int fail(ua_ctx *c, int code, const char *fmt, ...) {
     return code;
 }
 
-#define HIPCHK(c, expr)                                                                          \
-    do {                                                                                         \
-        hipError_t e__ = (expr);                                                                 \
-        if (e__ != hipSuccess)                                                                   \
-            return fail(c, e__ == hipErrorOutOfMemory ? UA_ERR_OOM : UA_ERR_HIP, "%s: %s (%s:%d)", \
-                        #expr, hipGetErrorString(e__), __FILE__, __LINE__);                      \
-    } while (0)
-
 int status_of(const QueryResult &r) {
     if (r.status == ST_OK) return r.found ? UA_OK : UA_NO_PATH;
     return r.status;
 }
 
 // ---- per-query planning -------------------------------------------------------
-
-struct Plan {
-    size_t best_bytes = 0, vals_bytes = 0, heap_bytes = 0;
-    size_t total() const { return best_bytes + vals_bytes + heap_bytes; }
-};
 
 int grid_fill_desc(ua_ctx *c, const ua_grid *g, int sx, int sy, int ex, int ey, int heaps, int heap_scale,
                    QueryDesc &q, Plan &p) {
@@ -213,7 +128,7 @@ int puzzle_fill_desc(ua_ctx *c, int n, const uint8_t *tiles, int heaps, int hash
     return UA_OK;
 }
 
-uint64_t pdb_bytes(int n) { return n == 4 ? 576576000ull : 362880ull; }
+static uint64_t pdb_bytes(int n) { return n == 4 ? 576576000ull : 362880ull; }
 
 int ensure_pdb(ua_ctx *c, int n) {
     if (n != 3 && n != 4) return fail(c, UA_ERR_INVALID, "puzzle side must be 3 or 4");
@@ -255,7 +170,7 @@ void fill_result(const QueryDesc &q, const QueryResult &r, ua_result *o) {
 
 // ---- plain solve (whole GPU, cooperative launch) ----------------------------------
 
-int plain_heaps(ua_ctx *c) {
+static int plain_heaps(ua_ctx *c) {
     int per_cu = 0;
     if (teams_max_blocks(&per_cu) != hipSuccess || per_cu < 1) per_cu = 1;
     int want = c->cfg.plain_heaps > 0 ? c->cfg.plain_heaps : 4 * c->num_cu;
@@ -264,7 +179,7 @@ int plain_heaps(ua_ctx *c) {
     return blocks * 4;
 }
 
-int run_wide(ua_ctx *c, QueryDesc &q, const Plan &p) {
+static int run_wide(ua_ctx *c, QueryDesc &q, const Plan &p) {
     int heaps = q.heaps;
     int blocks = heaps / 4;
     int segcap = heaps * 64 * (q.kind == KIND_GRID ? 8 : 4);
@@ -289,9 +204,15 @@ int run_wide(ua_ctx *c, QueryDesc &q, const Plan &p) {
     HIPCHK(c, c->qh.ensure(64));
     HIPCHK(c, hipMemsetAsync(c->qh.p, 0, 64, c->st));
     HIPCHK(c, hipEventRecord(c->ev0, c->st));
-    HIPCHK(c, launch_teams(c->qd.as<QueryDesc>(), c->qr.as<QueryResult>(), 1, c->qh.as<uint32_t>(),
-                           c->tv.as<TeamVars>(), c->seg.as<uint64_t>(), segcap, blocks, c->pdb[3].as<uint8_t>(),
-                           c->pdb[4].as<uint8_t>(), blocks, c->st));
+    LaunchShape shape{};
+    shape.groups[0] = 1;
+    shape.bpg[0] = blocks;
+    shape.nq[0] = 1;
+    shape.segcap[0] = segcap;
+    shape.bpg[1] = 1;
+    HIPCHK(c, launch_teams(c->qd.as<QueryDesc>(), c->qr.as<QueryResult>(), shape, c->qh.as<uint32_t>(),
+                           c->tv.as<TeamVars>(), c->seg.as<uint64_t>(), c->pdb[3].as<uint8_t>(),
+                           c->pdb[4].as<uint8_t>(), c->st));
     HIPCHK(c, hipEventRecord(c->ev1, c->st));
     QueryResult r{};
     HIPCHK(c, hipMemcpyAsync(&r, c->qr.p, sizeof r, hipMemcpyDeviceToHost, c->st));
@@ -305,7 +226,7 @@ int run_wide(ua_ctx *c, QueryDesc &q, const Plan &p) {
     return UA_OK;
 }
 
-}  // namespace
+}  // namespace uart
 
 // =================================================================== C ABI
 
@@ -530,55 +451,89 @@ extern "C" int ua_puzzle_moves(ua_ctx *c, int32_t *dirs, int64_t cap, int64_t *l
 
 // ------------------------------------------------------------------ batch
 
-namespace {
+namespace uart {
 
-// Waves per batched query: blocks of 4 waves, `team_waves` rounded up to whole blocks.
-int batch_bpg(ua_ctx *c) {
-    int w = c->cfg.team_waves > 0 ? c->cfg.team_waves : 4;
-    return std::min(8, std::max(1, (w + 3) / 4));
+// Blocks (of 4 waves) per batched / served query team, by kind: grids are path-length
+// bound and run best on small teams; puzzles are frontier-wide and shallow and run
+// best on wide teams.
+int team_bpg(ua_ctx *c, int kind) {
+    int w = kind == KIND_PUZZLE ? (c->cfg.puzzle_team_waves > 0 ? c->cfg.puzzle_team_waves : 64)
+                                : (c->cfg.team_waves > 0 ? c->cfg.team_waves : 4);
+    return std::min(64, std::max(1, (w + 3) / 4));
 }
 
-// Runs queries [descs] (already planned, arena offsets assigned) in one cooperative
-// launch of k_teams with one group of `bpg` blocks per concurrently running query.
-int run_batch(ua_ctx *c, std::vector<QueryDesc> &descs, std::vector<QueryResult> &res, int bpg) {
-    int nq = (int)descs.size();
+// Runs the planned queries in one cooperative launch of k_teams: class 0 = puzzles on
+// wide groups, class 1 = grids on one-block groups (each desc's `heaps` was planned
+// for its class). Results come back in the order of descs.
+int run_batch(ua_ctx *c, std::vector<QueryDesc> &descs, std::vector<QueryResult> &res) {
+    const int nq = (int)descs.size();
     res.assign(nq, QueryResult{});
     if (!nq) return UA_OK;
-    const int waves = 4 * bpg;
-    const int segcap = waves * 64 * 8;
+    std::vector<int> order;
+    order.reserve(nq);
+    for (int pass = 0; pass < 2; ++pass)
+        for (int i = 0; i < nq; ++i)
+            if ((descs[i].kind == KIND_PUZZLE) == (pass == 0)) order.push_back(i);
+    int n0 = 0;
+    for (int i = 0; i < nq; ++i) n0 += descs[i].kind == KIND_PUZZLE;
+    const int n1 = nq - n0;
     int per_cu = 0;
     HIPCHK(c, teams_max_blocks(&per_cu));
-    int groups = std::max(1, std::min(nq, c->num_cu * std::max(1, per_cu) / bpg));
-    int blocks = groups * bpg;
+    const int resident = std::max(1, c->num_cu * std::max(1, per_cu));
+    LaunchShape shape{};
+    shape.bpg[0] = std::min(team_bpg(c, KIND_PUZZLE), resident);
+    shape.bpg[1] = std::min(team_bpg(c, KIND_GRID), resident);
+    for (int i = 0; i < nq; ++i) {
+        int want = 4 * (descs[i].kind == KIND_PUZZLE ? shape.bpg[0] : shape.bpg[1]);
+        if (descs[i].heaps != want) return fail(c, UA_ERR_INVALID, "query %d planned for %d heaps, launch has %d",
+                                                i, descs[i].heaps, want);
+    }
+    int blocks0 = 0;
+    if (n0) {
+        int share = n1 ? resident / 2 : resident;  // both classes present: split the device
+        blocks0 = std::max(shape.bpg[0], std::min(n0 * shape.bpg[0], share / shape.bpg[0] * shape.bpg[0]));
+        blocks0 = std::min(blocks0, resident / shape.bpg[0] * shape.bpg[0]);
+    }
+    shape.groups[0] = n0 ? blocks0 / shape.bpg[0] : 0;
+    shape.groups[1] = n1 ? std::max(1, std::min(n1, (resident - blocks0) / shape.bpg[1])) : 0;
+    shape.nq[0] = n0;
+    shape.nq[1] = n1;
+    shape.qoff[0] = 0;
+    shape.qoff[1] = n0;
+    shape.segcap[0] = 4 * shape.bpg[0] * 64 * 4;  // puzzles: at most 4 successors per pop
+    shape.segcap[1] = 4 * shape.bpg[1] * 64 * 8;  // grids: at most 8
+    const int groups = shape.groups[0] + shape.groups[1];
+    const size_t seg_words =
+        (size_t)shape.groups[0] * 2 * shape.segcap[0] + (size_t)shape.groups[1] * 2 * shape.segcap[1];
+    std::vector<QueryDesc> sorted(nq);
+    for (int k = 0; k < nq; ++k) sorted[k] = descs[order[k]];
     HIPCHK(c, c->b_qd.ensure(sizeof(QueryDesc) * nq));
     HIPCHK(c, c->b_qr.ensure(sizeof(QueryResult) * nq));
     HIPCHK(c, c->b_head.ensure(64));
     HIPCHK(c, c->b_tv.ensure(sizeof(TeamVars) * groups));
-    HIPCHK(c, c->b_seg.ensure((size_t)groups * 2 * segcap * 8));
-    HIPCHK(c, hipMemcpyAsync(c->b_qd.p, descs.data(), sizeof(QueryDesc) * nq, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, c->b_seg.ensure(seg_words * 8));
+    HIPCHK(c, hipMemcpyAsync(c->b_qd.p, sorted.data(), sizeof(QueryDesc) * nq, hipMemcpyHostToDevice, c->st));
     HIPCHK(c, hipMemsetAsync(c->b_head.p, 0, 64, c->st));
     HIPCHK(c, hipMemsetAsync(c->b_tv.p, 0, sizeof(TeamVars) * groups, c->st));
     HIPCHK(c, hipEventRecord(c->ev0, c->st));
-    HIPCHK(c, launch_teams(c->b_qd.as<QueryDesc>(), c->b_qr.as<QueryResult>(), nq, c->b_head.as<uint32_t>(),
-                           c->b_tv.as<TeamVars>(), c->b_seg.as<uint64_t>(), segcap, bpg, c->pdb[3].as<uint8_t>(),
-                           c->pdb[4].as<uint8_t>(), blocks, c->st));
+    HIPCHK(c, launch_teams(c->b_qd.as<QueryDesc>(), c->b_qr.as<QueryResult>(), shape, c->b_head.as<uint32_t>(),
+                           c->b_tv.as<TeamVars>(), c->b_seg.as<uint64_t>(), c->pdb[3].as<uint8_t>(),
+                           c->pdb[4].as<uint8_t>(), c->st));
     HIPCHK(c, hipEventRecord(c->ev1, c->st));
-    HIPCHK(c, hipMemcpyAsync(res.data(), c->b_qr.p, sizeof(QueryResult) * nq, hipMemcpyDeviceToHost, c->st));
+    std::vector<QueryResult> sres(nq);
+    HIPCHK(c, hipMemcpyAsync(sres.data(), c->b_qr.p, sizeof(QueryResult) * nq, hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
+    for (int k = 0; k < nq; ++k) res[order[k]] = sres[k];
     float kms = 0;
     HIPCHK(c, hipEventElapsedTime(&kms, c->ev0, c->ev1));
     c->last_kernel_ms = kms;
     return UA_OK;
 }
 
-size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
-
-}  // namespace
+}  // namespace uart
 
 extern "C" int ua_batch_solve(ua_ctx *c, const ua_query *qs, int64_t nq, ua_result *out) {
     if (!c || (!qs && nq) || (!out && nq) || nq < 0) return fail(c, UA_ERR_INVALID, "null argument");
-    const int bpg = batch_bpg(c);
-    const int waves = 4 * bpg;
     size_t free_b = 0, total_b = 0;
     HIPCHK(c, hipMemGetInfo(&free_b, &total_b));
     size_t budget = c->cfg.mem_budget ? c->cfg.mem_budget : (size_t)(free_b * 0.85);
@@ -591,13 +546,14 @@ extern "C" int ua_batch_solve(ua_ctx *c, const ua_query *qs, int64_t nq, ua_resu
         const ua_query &u = qs[i];
         int rc;
         if (u.kind == UA_QUERY_GRID) {
-            rc = grid_fill_desc(c, u.grid, u.sx, u.sy, u.ex, u.ey, waves, 1, descs[i], plans[i]);
+            rc = grid_fill_desc(c, u.grid, u.sx, u.sy, u.ex, u.ey, 4 * team_bpg(c, KIND_GRID), 1, descs[i],
+                                plans[i]);
         } else if (u.kind == UA_QUERY_PUZZLE) {
             if (!board_ok(u.n, u.tiles)) rc = fail(c, UA_ERR_INVALID, "query %lld: bad board", (long long)i);
             else if (!board_solvable(u.n, u.tiles)) rc = UA_NO_PATH;
             else if ((rc = ensure_pdb(c, u.n)) == UA_OK)
-                rc = puzzle_fill_desc(c, u.n, u.tiles, waves,
-                                      c->cfg.hash_log2 > 0 ? c->cfg.hash_log2 : (u.n == 3 ? 17 : 21), descs[i],
+                rc = puzzle_fill_desc(c, u.n, u.tiles, 4 * team_bpg(c, KIND_PUZZLE),
+                                      c->cfg.hash_log2 > 0 ? c->cfg.hash_log2 : (u.n == 3 ? 17 : 24), descs[i],
                                       plans[i]);
         } else {
             rc = fail(c, UA_ERR_INVALID, "query %lld: bad kind", (long long)i);
@@ -683,7 +639,7 @@ extern "C" int ua_batch_solve(ua_ctx *c, const ua_query *qs, int64_t nq, ua_resu
             HIPCHK(c, hipMemsetAsync(base, 0xFF, ff_end, c->st));
             if (zero_end > ff_end) HIPCHK(c, hipMemsetAsync(base + ff_end, 0, zero_end - ff_end, c->st));
             std::vector<QueryResult> sr;
-            int rc = run_batch(c, sub, sr, bpg);
+            int rc = run_batch(c, sub, sr);
             if (rc) return rc;
             for (size_t k = pos; k < end; ++k) {
                 int64_t i = todo[k];
@@ -698,10 +654,11 @@ extern "C" int ua_batch_solve(ua_ctx *c, const ua_query *qs, int64_t nq, ua_resu
             scale[i] *= 4;
             if (descs[i].kind == KIND_GRID) {
                 const ua_query &u = qs[i];
-                grid_fill_desc(c, u.grid, u.sx, u.sy, u.ex, u.ey, waves, scale[i], descs[i], plans[i]);
+                grid_fill_desc(c, u.grid, u.sx, u.sy, u.ex, u.ey, 4 * team_bpg(c, KIND_GRID), scale[i], descs[i],
+                               plans[i]);
             } else {
                 int hl = descs[i].hash_log2 + 2;
-                puzzle_fill_desc(c, qs[i].n, qs[i].tiles, waves, hl, descs[i], plans[i]);
+                puzzle_fill_desc(c, qs[i].n, qs[i].tiles, 4 * team_bpg(c, KIND_PUZZLE), hl, descs[i], plans[i]);
             }
         }
         todo.swap(retry);

@@ -55,10 +55,10 @@ struct GroupTeam {
     int bpg;        // blocks per group
     TeamMem m;
     TeamVars *tv;
-    __device__ GroupTeam(const TeamMem &mem, TeamVars *t, int blocks_per_group)
+    __device__ GroupTeam(const TeamMem &mem, TeamVars *t, int blocks_per_group, int block_in_group)
         : m(mem), tv(t) {
         bpg = blocks_per_group;
-        gblock = (int)(blockIdx.x % (unsigned)bpg);
+        gblock = block_in_group;
         int wpb = (int)(blockDim.x >> 6);
         tw = __builtin_amdgcn_readfirstlane(gblock * wpb + (int)(threadIdx.x >> 6));
         W = bpg * wpb;
@@ -72,6 +72,13 @@ struct GroupTeam {
         constexpr int SC = SLOT_SCOPE;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         __syncthreads();
+        if (bpg == 1) {  // a one-block group needs no arrival counter: same fences, no atomics
+            if (threadIdx.x == 0) hook();
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            __syncthreads();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            return true;
+        }
         __shared__ uint32_t s_ok;
         if (threadIdx.x == 0) {
             uint32_t ok = 1;
@@ -360,6 +367,9 @@ __device__ void team_solve(const QueryDesc &q, QueryResult *res, const Team &tea
 
     WaveHeap heap;
     heap.init(q.heap + (size_t)tw * (size_t)(q.heap_cap - 1) * 64, q.heap_cap);
+    // a query suspended by an earlier serving launch resumes from its record
+    const bool resumed = q.resume != nullptr && q.resume[0] != 0;
+    uint64_t *rec_heap = q.resume ? q.resume + RESUME_HEADER + (size_t)tw * RESUME_PER_HEAP : nullptr;
     if (team.leader()) {
         for (int i = 0; i < 2; ++i) {
             __hip_atomic_store(&slots[i].lb, SENT, __ATOMIC_RELAXED, SS);
@@ -367,9 +377,15 @@ __device__ void team_solve(const QueryDesc &q, QueryResult *res, const Team &tea
             __hip_atomic_store(&slots[i].cnt, 0u, __ATOMIC_RELAXED, SS);
             __hip_atomic_store(&slots[i].err, 0u, __ATOMIC_RELAXED, SS);
         }
-        for (int i = 0; i < 4; ++i) __hip_atomic_store(&stats[i], (uint64_t)0, __ATOMIC_RELAXED, SS);
+        for (int i = 0; i < 4; ++i)
+            __hip_atomic_store(&stats[i], resumed ? q.resume[3 + i] : (uint64_t)0, __ATOMIC_RELAXED, SS);
     }
-    if (tw == 0) {
+    if (resumed) {
+        heap.root = rec_heap[l];
+        heap.part = rec_heap[64 + l];
+        heap.n = (int)rec_heap[128];
+        heap.pc = (int)rec_heap[129];
+    } else if (tw == 0) {
         uint64_t it = SENT;
         if (l == 0) it = P.template seed<DS>();
         heap.part = it;
@@ -378,10 +394,11 @@ __device__ void team_solve(const QueryDesc &q, QueryResult *res, const Team &tea
     if (!team.sync([] {})) return;
 
     int64_t n_pop = 0, n_exp = 0, n_gen = 0, n_ins = 0;
-    uint64_t best_goal = SENT;
+    uint64_t best_goal = resumed ? q.resume[2] : SENT;
     int32_t status = ST_OK;
     int found = 0;
-    int r = 0;
+    const int r0 = resumed ? (int)q.resume[1] : 0;
+    int r = r0;
     for (;; ++r) {
         RoundSlot *cur = &slots[r & 1];
         uint64_t *cur_seg = team.m.seg + (size_t)(r & 1) * segcap;
@@ -459,6 +476,18 @@ __device__ void team_solve(const QueryDesc &q, QueryResult *res, const Team &tea
             fits &= heap.insert(v, sreal);
         }
         if (!fits && l == 0) __hip_atomic_fetch_or(&nxt->err, FLAG_CAPACITY, __ATOMIC_RELAXED, SS);
+        if (q.round_budget > 0 && r + 1 - r0 >= q.round_budget && q.resume != nullptr) {
+            status = ST_SUSPENDED;  // every wave reaches this with the same r
+            break;
+        }
+    }
+    if (status == ST_SUSPENDED) {
+        rec_heap[l] = heap.root;
+        rec_heap[64 + l] = heap.part;
+        if (l == 0) {
+            rec_heap[128] = (uint64_t)heap.n;
+            rec_heap[129] = (uint64_t)heap.pc;
+        }
     }
 
     // ---- statistics and result
@@ -474,6 +503,12 @@ __device__ void team_solve(const QueryDesc &q, QueryResult *res, const Team &tea
     }
     team.sync([] {});
     if (team.leader()) {
+        if (status == ST_SUSPENDED) {
+            q.resume[1] = (uint64_t)(r + 1);
+            q.resume[2] = best_goal;
+            for (int i = 0; i < 4; ++i) q.resume[3 + i] = __hip_atomic_load(&stats[i], __ATOMIC_RELAXED, SS);
+            q.resume[0] = 1;
+        }
         res->status = status;
         res->found = found;
         res->goal = best_goal;
@@ -503,31 +538,43 @@ __device__ void dispatch(const QueryDesc &q, QueryResult *res, const uint8_t *pd
 
 __device__ __forceinline__ int64_t now_ticks() { return (int64_t)__builtin_amdgcn_s_memrealtime(); }
 
-// Body of the grouped cooperative kernel for one block: the group leader takes the
-// next query index from the shared counter and hands it to the group through the
-// group's TeamVars; the group solves it as one team; repeat until the list is
-// exhausted. The index is wave-uniform, so descriptors are read with scalar loads.
-__device__ inline void group_block(const QueryDesc *qs, QueryResult *rs, int nq, uint32_t *qhead,
-                                   TeamVars *tvs, uint64_t *segs, int segcap, int bpg, const uint8_t *pdb3,
+// Body of k_teams for one block: find this block's class and group from the launch
+// shape; the group leader takes the next query index of its class and hands it to
+// the group through the group's TeamVars; the group solves it as one team; repeat
+// until the class's queries are exhausted. The index is wave-uniform, so query
+// descriptors are read with scalar loads.
+__device__ inline void group_block(const QueryDesc *qs, QueryResult *rs, const LaunchShape &shape,
+                                   uint32_t *qhead, TeamVars *tvs, uint64_t *segs, const uint8_t *pdb3,
                                    const uint8_t *pdb4) {
-    const int group = (int)(blockIdx.x / (unsigned)bpg);
+    const int b = (int)blockIdx.x;
+    const int first1 = shape.groups[0] * shape.bpg[0];  // first block of class 1
+    const int cls = b < first1 ? 0 : 1;
+    const int rel = cls ? b - first1 : b;
+    const int bpg = shape.bpg[cls];
+    const int group = (cls ? shape.groups[0] : 0) + rel / bpg;
+    const int segcap = shape.segcap[cls];
+    size_t seg_off = cls ? (size_t)shape.groups[0] * 2 * shape.segcap[0] + (size_t)(rel / bpg) * 2 * segcap
+                         : (size_t)(rel / bpg) * 2 * segcap;
     TeamVars *tv = tvs + group;
     TeamMem m;
     m.slot = tv->slot;
     m.stats = &tv->n_pop;
-    m.seg = segs + (size_t)group * 2 * segcap;
+    m.seg = segs + seg_off;
     m.segcap = segcap;
-    GroupTeam team(m, tv, bpg);
+    GroupTeam team(m, tv, bpg, rel % bpg);
+    uint32_t *head = qhead + cls;
+    const int nq = shape.nq[cls], qoff = shape.qoff[cls];
     for (;;) {
         int64_t t0 = now_ticks();
-        if (!team.sync([tv, qhead] {
-                uint32_t qi = atomicAdd(qhead, 1u);
+        if (!team.sync([tv, head] {
+                uint32_t qi = atomicAdd(head, 1u);
                 __hip_atomic_store(&tv->next_q, qi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }))
             break;
-        int qi = __builtin_amdgcn_readfirstlane(
+        int k = __builtin_amdgcn_readfirstlane(
             (int)__hip_atomic_load(&tv->next_q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-        if (qi >= nq) break;
+        if (k >= nq) break;
+        int qi = qoff + k;
         QueryDesc q = qs[qi];
         dispatch(q, rs + qi, pdb3, pdb4, team);
         if (team.leader()) {

@@ -1,10 +1,342 @@
-// serve.cpp -- continuous-batching request loop (see ua_serve_* in include/uastar.h).
-#include <hip/hip_runtime.h>
-#include "../../include/uastar.h"
+// serve.cpp -- continuous batching of grid and puzzle queries (ua_serve_* in include/uastar.h).
+//
+// Submitted queries wait in a pending queue. Each ua_serve_step():
+//   1. admits pending queries while a team slot is free and their memory estimate
+//      (best-cost table or hash, heap storage, resume record) fits the device pool;
+//   2. runs ONE bounded cooperative launch of k_teams over every resident query:
+//      each query advances up to `round_budget` GA* rounds, then saves its heap tops
+//      and counters to its resume record (state stays resident in HBM);
+//   3. retires the queries that finished (found / no path / error), frees their memory.
+// Queries that arrive while others are on the device are admitted at the next step,
+// so their work overlaps on the device with the queries already in flight. The round
+// budget adapts so that one launch lasts a few milliseconds. A query that outgrows
+// its heap or hash is restarted with a larger plan.
+#include <algorithm>
+#include <chrono>
+#include <deque>
+#include <map>
 
-struct ua_ctx;
-void ua_serve_release(ua_ctx *) {}
-extern "C" int ua_serve_submit(ua_ctx *, const ua_query *, uint64_t *) { return UA_ERR_INVALID; }
-extern "C" int ua_serve_step(ua_ctx *, int64_t *, int64_t *) { return UA_ERR_INVALID; }
-extern "C" int ua_serve_poll(ua_ctx *, uint64_t *, ua_result *, int64_t, int64_t *) { return UA_ERR_INVALID; }
-extern "C" int ua_serve_drain(ua_ctx *) { return UA_ERR_INVALID; }
+#include "internal.hpp"
+
+using namespace ua;
+using namespace uart;
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+// First-fit allocator over one device arena (offsets, 256-byte granules).
+struct Pool {
+    DevBuf arena;
+    size_t cap = 0;
+    std::map<size_t, size_t> free_;  // offset -> size
+    hipError_t init(size_t bytes) {
+        bytes &= ~(size_t)255;
+        hipError_t e = arena.ensure(bytes);
+        if (e != hipSuccess) return e;
+        cap = bytes;
+        free_.clear();
+        free_[0] = bytes;
+        return hipSuccess;
+    }
+    bool alloc(size_t n, size_t *off) {
+        n = align256(n);
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < n) continue;
+            *off = it->first;
+            size_t rest = it->second - n, start = it->first + n;
+            free_.erase(it);
+            if (rest) free_[start] = rest;
+            return true;
+        }
+        return false;
+    }
+    void release(size_t off, size_t n) {
+        n = align256(n);
+        auto it = free_.emplace(off, n).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+};
+
+struct Job {
+    uint64_t ticket = 0;
+    ua_query q{};
+    Clock::time_point submit;
+    QueryDesc d{};
+    Plan plan;
+    size_t off = 0, bytes = 0;
+    int scale = 1;      // grid heap scale / puzzle hash growth steps
+    int launches = 0;
+    int64_t rounds = 0;  // rounds completed so far
+    double device_ms = 0, kernel_ms = 0;
+};
+
+}  // namespace
+
+struct ServeEngine {
+    std::mutex mu;  // guards pending, done, next_ticket
+    std::deque<Job> pending;
+    std::vector<Job> active;
+    std::vector<std::pair<uint64_t, ua_result>> done;
+    uint64_t next_ticket = 1;
+    Pool pool;
+    bool pool_ready = false;
+    int round_budget[2] = {32, 128};  // per class: 0 = puzzles, 1 = grids
+    int64_t launches = 0;
+    double kernel_ms = 0;
+};
+
+static int cls_of(const ua_query &q) { return q.kind == UA_QUERY_PUZZLE ? 0 : 1; }
+
+void ua_serve_release(ua_ctx *c) {
+    if (!c || !c->serve) return;
+    c->serve->pool.arena.release();
+    delete c->serve;
+    c->serve = nullptr;
+}
+
+namespace {
+
+ServeEngine *engine(ua_ctx *c) {
+    std::lock_guard<std::mutex> g(c->mu);
+    if (!c->serve) c->serve = new ServeEngine();
+    return c->serve;
+}
+
+// Plans job j for its current scale and sets its device pointers inside the pool.
+int plan_job(ua_ctx *c, ServeEngine *e, Job &j, bool *fits) {
+    *fits = false;
+    const int heaps = 4 * team_bpg(c, j.q.kind == UA_QUERY_PUZZLE ? KIND_PUZZLE : KIND_GRID);
+    int rc;
+    if (j.q.kind == UA_QUERY_GRID) {
+        rc = grid_fill_desc(c, j.q.grid, j.q.sx, j.q.sy, j.q.ex, j.q.ey, heaps, j.scale, j.d, j.plan);
+    } else {
+        rc = ensure_pdb(c, j.q.n);
+        if (!rc) {
+            int base = c->cfg.hash_log2 > 0 ? c->cfg.hash_log2 : (j.q.n == 3 ? 17 : 24);
+            int grow = 0;
+            for (int s = j.scale; s > 1; s /= 4) grow += 2;
+            rc = puzzle_fill_desc(c, j.q.n, j.q.tiles, heaps, base + grow, j.d, j.plan);
+        }
+    }
+    if (rc) return rc;
+    size_t rec = align256((size_t)resume_words(heaps) * 8);
+    j.bytes = align256(j.plan.best_bytes) + align256(j.plan.vals_bytes) + align256(j.plan.heap_bytes) + rec;
+    if (j.bytes > e->pool.cap) return fail(c, UA_ERR_OOM, "query needs %zu bytes, pool is %zu", j.bytes, e->pool.cap);
+    size_t off;
+    if (!e->pool.alloc(j.bytes, &off)) return UA_OK;  // wait for memory
+    *fits = true;
+    j.off = off;
+    char *base = e->pool.arena.as<char>() + off;
+    size_t o = 0;
+    j.d.best = reinterpret_cast<uint64_t *>(base + o);
+    o += align256(j.plan.best_bytes);
+    j.d.vals = j.plan.vals_bytes ? reinterpret_cast<uint32_t *>(base + o) : nullptr;
+    o += align256(j.plan.vals_bytes);
+    j.d.heap = reinterpret_cast<uint64_t *>(base + o);
+    o += align256(j.plan.heap_bytes);
+    j.d.resume = reinterpret_cast<uint64_t *>(base + o);
+    // fresh search state: grid best = all ones; puzzle keys = 0, vals = all ones; no resume record
+    if (j.d.kind == KIND_GRID) {
+        HIPCHK(c, hipMemsetAsync(j.d.best, 0xFF, j.plan.best_bytes, c->st));
+    } else {
+        HIPCHK(c, hipMemsetAsync(j.d.best, 0, j.plan.best_bytes, c->st));
+        HIPCHK(c, hipMemsetAsync(j.d.vals, 0xFF, j.plan.vals_bytes, c->st));
+    }
+    HIPCHK(c, hipMemsetAsync(j.d.resume, 0, 64, c->st));
+    return UA_OK;
+}
+
+void complete(ServeEngine *e, Job &j, const ua_result &r) {
+    std::lock_guard<std::mutex> g(e->mu);
+    e->done.emplace_back(j.ticket, r);
+}
+
+}  // namespace
+
+extern "C" int ua_serve_submit(ua_ctx *c, const ua_query *q, uint64_t *ticket) {
+    if (!c || !q) return fail(c, UA_ERR_INVALID, "null argument");
+    ServeEngine *e = engine(c);
+    Job j;
+    j.q = *q;
+    j.submit = Clock::now();
+    if (q->kind == UA_QUERY_GRID) {
+        if (!q->grid) return fail(c, UA_ERR_INVALID, "grid query without a grid");
+    } else if (q->kind == UA_QUERY_PUZZLE) {
+        if (!board_ok(q->n, q->tiles)) return fail(c, UA_ERR_INVALID, "board is not a permutation");
+    } else {
+        return fail(c, UA_ERR_INVALID, "bad query kind");
+    }
+    std::lock_guard<std::mutex> g(e->mu);
+    j.ticket = e->next_ticket++;
+    if (ticket) *ticket = j.ticket;
+    if (q->kind == UA_QUERY_PUZZLE && !board_solvable(q->n, q->tiles)) {
+        ua_result r{};
+        r.status = UA_NO_PATH;
+        e->done.emplace_back(j.ticket, r);
+        return UA_OK;
+    }
+    e->pending.push_back(j);
+    return UA_OK;
+}
+
+extern "C" int ua_serve_step(ua_ctx *c, int64_t *n_active, int64_t *n_pending) {
+    if (!c) return UA_ERR_INVALID;
+    ServeEngine *e = engine(c);
+    if (!e->pool_ready) {
+        size_t free_b = 0, total_b = 0;
+        HIPCHK(c, hipMemGetInfo(&free_b, &total_b));
+        size_t budget = c->cfg.mem_budget ? c->cfg.mem_budget : (size_t)(free_b * 0.80);
+        HIPCHK(c, e->pool.init(budget));
+        e->pool_ready = true;
+    }
+    int per_cu = 0;
+    HIPCHK(c, teams_max_blocks(&per_cu));
+    const int resident = c->num_cu * std::max(1, per_cu);
+    // resident queries per class: about two per group the launch can hold
+    const size_t cap_cls[2] = {(size_t)std::max(1, 2 * resident / 2 / team_bpg(c, KIND_PUZZLE)),
+                               (size_t)std::max(1, 2 * resident / team_bpg(c, KIND_GRID))};
+    size_t n_cls[2] = {0, 0};
+    for (const Job &j : e->active) n_cls[cls_of(j.q)]++;
+
+    // 1. admission (in arrival order; a class at its cap waits, the other can still be admitted)
+    {
+        std::deque<Job> later;
+        for (;;) {
+            Job j;
+            {
+                std::lock_guard<std::mutex> g(e->mu);
+                if (e->pending.empty()) break;
+                j = e->pending.front();
+                e->pending.pop_front();
+            }
+            int k = cls_of(j.q);
+            if (n_cls[k] >= cap_cls[k]) {
+                later.push_back(j);
+                continue;
+            }
+            bool fits = false;
+            int rc = plan_job(c, e, j, &fits);
+            if (rc) {  // cannot ever run: report it
+                ua_result r{};
+                r.status = rc;
+                complete(e, j, r);
+                continue;
+            }
+            if (!fits) {  // wait for memory to free up
+                later.push_back(j);
+                break;
+            }
+            n_cls[k]++;
+            e->active.push_back(j);
+        }
+        std::lock_guard<std::mutex> g(e->mu);
+        for (auto it = later.rbegin(); it != later.rend(); ++it) e->pending.push_front(*it);
+    }
+    if (!e->active.empty()) {
+        // 2. one bounded launch over every resident query
+        std::vector<QueryDesc> descs;
+        descs.reserve(e->active.size());
+        for (Job &j : e->active) {
+            j.d.round_budget = e->round_budget[cls_of(j.q)];
+            descs.push_back(j.d);
+        }
+        std::vector<QueryResult> res;
+        int rc = run_batch(c, descs, res);
+        if (rc) return rc;
+        double launch_ms = c->last_kernel_ms;
+        e->launches++;
+        e->kernel_ms += launch_ms;
+        // 3. retirement
+        std::vector<Job> still;
+        auto now = Clock::now();
+        double cls_ms[2] = {0, 0}, cls_rounds[2] = {0, 0};
+        for (size_t i = 0; i < e->active.size(); ++i) {
+            Job &j = e->active[i];
+            const QueryResult &r = res[i];
+            const double dms = (double)(r.t_end - r.t_start) / 1e5;
+            j.launches++;
+            j.kernel_ms += launch_ms;
+            j.device_ms += dms;
+            cls_ms[cls_of(j.q)] += dms;
+            cls_rounds[cls_of(j.q)] += (double)(r.rounds - j.rounds);
+            j.rounds = r.rounds;
+            if (r.status == ST_SUSPENDED) {
+                still.push_back(j);
+                continue;
+            }
+            e->pool.release(j.off, j.bytes);
+            if (r.status == ST_CAPACITY && j.scale < 256) {  // restart with a larger plan
+                j.scale *= 4;
+                j.rounds = 0;
+                std::lock_guard<std::mutex> g(e->mu);
+                e->pending.push_front(j);
+                continue;
+            }
+            ua_result out;
+            fill_result(j.d, r, &out);
+            out.kernel_ms = j.kernel_ms;
+            out.device_ms = j.device_ms;
+            out.latency_ms = std::chrono::duration<double, std::milli>(now - j.submit).count();
+            complete(e, j, out);
+        }
+        e->active.swap(still);
+        // keep each class's share of a launch near 4 ms: budget = 4 ms / (ms per round)
+        for (int k = 0; k < 2; ++k) {
+            if (cls_rounds[k] <= 0) continue;
+            double ms_per_round = cls_ms[k] / cls_rounds[k];
+            int want = (int)std::min(65536.0, std::max(8.0, 4.0 / std::max(ms_per_round, 1e-4)));
+            e->round_budget[k] = (e->round_budget[k] + want) / 2;
+        }
+    }
+    if (n_active) *n_active = (int64_t)e->active.size();
+    if (n_pending) {
+        std::lock_guard<std::mutex> g(e->mu);
+        *n_pending = (int64_t)e->pending.size();
+    }
+    return UA_OK;
+}
+
+extern "C" int ua_serve_poll(ua_ctx *c, uint64_t *tickets, ua_result *results, int64_t cap, int64_t *n_out) {
+    if (!c || !n_out || cap < 0) return fail(c, UA_ERR_INVALID, "bad arguments");
+    ServeEngine *e = engine(c);
+    std::lock_guard<std::mutex> g(e->mu);
+    int64_t n = std::min<int64_t>(cap, (int64_t)e->done.size());
+    for (int64_t i = 0; i < n; ++i) {
+        if (tickets) tickets[i] = e->done[i].first;
+        if (results) results[i] = e->done[i].second;
+    }
+    e->done.erase(e->done.begin(), e->done.begin() + n);
+    *n_out = n;
+    return UA_OK;
+}
+
+extern "C" int ua_serve_drain(ua_ctx *c) {
+    if (!c) return UA_ERR_INVALID;
+    for (;;) {
+        int64_t a = 0, p = 0;
+        int rc = ua_serve_step(c, &a, &p);
+        if (rc) return rc;
+        if (!a && !p) return UA_OK;
+    }
+}
+
+extern "C" int ua_serve_stats(ua_ctx *c, int64_t *launches, double *kernel_ms, int32_t *round_budget) {
+    if (!c) return UA_ERR_INVALID;
+    ServeEngine *e = engine(c);
+    if (launches) *launches = e->launches;
+    if (kernel_ms) *kernel_ms = e->kernel_ms;
+    if (round_budget) *round_budget = e->round_budget[1];
+    return UA_OK;
+}
