@@ -2,6 +2,7 @@
 
     python tests/golden/make_golden.py pdb      # PDB digests (sha256 + depth histogram)
     python tests/golden/make_golden.py c3       # 256 random solvable 15-puzzles, seeds 3000..3255
+    python tests/golden/make_golden.py c4       # 512 random solvable 15-puzzles, seeds 41000..41511
     python tests/golden/make_golden.py grids    # seeded grid configs (1000^2 seed 1; 16384^2 seed 2)
 
 The oracle itself is pinned by tests/test_oracle.py (closed forms, exhaustive BFS,
@@ -59,17 +60,22 @@ def _solve_seed(seed):
             "cpu_s": round(time.time() - t, 3)}
 
 
-def cmd_c3(workers=6):
+def cmd_c3(workers=6, seeds=None, out_name="c3_puzzles.json"):
     O.pdb_full(4, 1, cache_dir=CACHE)
-    seeds = list(range(3000, 3256))
+    seeds = seeds or list(range(3000, 3256))
     with mp.Pool(workers) as pool:
         rows = []
         for r in pool.imap_unordered(_solve_seed, seeds):
             rows.append(r)
             print(r["seed"], r["steps"], r["expanded"], r["cpu_s"], flush=True)
     rows.sort(key=lambda r: r["seed"])
-    with open(os.path.join(HERE, "c3_puzzles.json"), "w") as f:
+    with open(os.path.join(HERE, out_name), "w") as f:
         json.dump(rows, f)
+
+
+def cmd_c4(workers=5):
+    """The 512 puzzles of the batched configuration: boards from seeds 41000..41511."""
+    cmd_c3(workers, list(range(41000, 41512)), "c4_puzzles.json")
 
 
 def cmd_grids():
@@ -88,4 +94,4 @@ def cmd_grids():
 
 
 if __name__ == "__main__":
-    {"pdb": cmd_pdb, "c3": cmd_c3, "grids": cmd_grids}[sys.argv[1]]()
+    {"pdb": cmd_pdb, "c3": cmd_c3, "c4": cmd_c4, "grids": cmd_grids}[sys.argv[1]]()
