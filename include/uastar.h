@@ -130,6 +130,31 @@ int ua_serve_drain(ua_ctx *ctx);
 /* launches so far, their summed duration (HIP events) and the current round budget */
 int ua_serve_stats(ua_ctx *ctx, int64_t *launches, double *kernel_ms, int32_t *round_budget);
 
+/* ---- autotuning (no reference counterpart: uAstar fixes its launch shape and table
+ * sizes at compile time, pathway/GPU-solver.hpp:5-17 and puzzle/GPU-solver.cuh:12-22).
+ * ua_autotune runs a caller-given sample of queries under candidate settings and keeps,
+ * per instance class, the one with the lowest measured cost: the team width of batched
+ * and served grid and 15-puzzle queries, the 15-puzzle hash size (load factor) and the
+ * heap count of the plain whole-GPU solve. Nonzero ua_config fields are left alone.
+ * The sample is solved several times; the state of an earlier ua_batch_solve is freed. */
+typedef struct {
+    int32_t grid_team_waves;    /* waves (= heaps) per batched / served grid query */
+    int32_t puzzle_team_waves;  /* waves (= heaps) per batched / served 15-puzzle query */
+    int32_t puzzle_hash_log2;   /* hash slots (log2) per batched / served 15-puzzle query */
+    int32_t plain_grid_heaps;   /* heaps of the plain whole-GPU grid solve */
+    int32_t plain_puzzle_heaps; /* heaps of the plain whole-GPU 15-puzzle solve */
+    int32_t plain_hash_log2;    /* hash slots (log2) of a plain 15-puzzle solve */
+    int32_t trials;             /* settings measured by the last ua_autotune */
+    int32_t reserved;
+    double est_grid_ms;         /* estimated stream time of the sample's grids at the chosen width */
+    double est_puzzle_ms;       /* the same for its 15-puzzles */
+} ua_tuning;
+int ua_autotune(ua_ctx *ctx, const ua_query *sample, int64_t n, ua_tuning *out);
+/* the settings in effect (tuned, configured or default) */
+int ua_get_tuning(ua_ctx *ctx, ua_tuning *out);
+/* applies stored settings without measuring (e.g. from an earlier ua_autotune) */
+int ua_set_tuning(ua_ctx *ctx, const ua_tuning *t);
+
 /* ---- diagnostics */
 /* Runs a random insert/pop sequence on one wave's heap; see tests. */
 int ua_debug_heap_selftest(ua_ctx *ctx, const int32_t *ops, int nops, const uint64_t *keys, int64_t nkeys,

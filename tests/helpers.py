@@ -55,3 +55,43 @@ def goal_board(n):
 def random_grid(rng, H, W, p):
     m = rng.random((H, W)) >= p
     return m
+
+
+def check_grid_path_bitmap(path, bitmap: np.ndarray, H, W, sx, sy, ex, ey, orth, diag):
+    """check_grid_path for large grids, vectorised over a row-padded bitmap (1 bit per
+    cell, bit (y & 63) of word x*wpr + (y >> 6))."""
+    p = np.asarray(path, np.int64)
+    assert p[0] == sx * W + sy and p[-1] == ex * W + ey, "path endpoints"
+    x, y = p // W, p % W
+    dx, dy = np.diff(x), np.diff(y)
+    step = np.maximum(np.abs(dx), np.abs(dy))
+    assert np.all(step == 1), "non-adjacent step"
+    assert np.all((x >= 0) & (x < H) & (y >= 0) & (y < W))
+    bm = np.asarray(bitmap, np.uint64).reshape(H, -1)
+    xs, ys = x[:-1], y[:-1]
+    bits = (bm[xs, ys >> 6] >> (ys & 63).astype(np.uint64)) & np.uint64(1)
+    assert np.all(bits == 1), "step out of a blocked cell"
+    nd = int(np.sum(np.abs(dx) + np.abs(dy) == 2))
+    assert (len(p) - 1 - nd, nd) == (orth, diag), "path counts differ from the cost"
+
+
+def c4_grid_specs(n=4096, seed=4):
+    """The grids of the batched configuration: side floor(exp(U(ln 256, ln 4096))),
+    obstacles Bernoulli(p) with p ~ U(0.1, 0.4) from a per-grid seed, start and goal
+    drawn uniformly among open cells (by the generator's cell rule)."""
+    import math
+
+    import uastar_amd as U
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        side = int(math.floor(math.exp(rng.uniform(math.log(256), math.log(4096)))))
+        p = float(rng.uniform(0.1, 0.4))
+        gseed = int(rng.integers(1, 1 << 62))
+        pts = []
+        while len(pts) < 2:
+            x, y = int(rng.integers(side)), int(rng.integers(side))
+            if U.cell_open(gseed, p, side, x, y):
+                pts.append((x, y))
+        out.append(dict(H=side, W=side, seed=gseed, p=p, s=pts[0], g=pts[1]))
+    return out

@@ -55,6 +55,16 @@ class Query(C.Structure):
                 ("sy", C.c_int32), ("ex", C.c_int32), ("ey", C.c_int32), ("tiles", C.c_uint8 * 16)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("grid_team_waves", C.c_int32), ("puzzle_team_waves", C.c_int32), ("puzzle_hash_log2", C.c_int32),
+                ("plain_grid_heaps", C.c_int32), ("plain_puzzle_heaps", C.c_int32), ("plain_hash_log2", C.c_int32),
+                ("trials", C.c_int32), ("reserved", C.c_int32), ("est_grid_ms", C.c_double),
+                ("est_puzzle_ms", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
+
+
 _lib = None
 
 
@@ -93,6 +103,9 @@ def lib():
             "ua_serve_stats": (C.c_int, [vp, C.POINTER(i64), C.POINTER(C.c_double), C.POINTER(i32)]),
             "ua_debug_heap_selftest": (C.c_int, [vp, vp, C.c_int, vp, i64, C.c_int, vp, i64,
                                                  C.POINTER(i32)]),
+            "ua_autotune": (C.c_int, [vp, C.POINTER(Query), i64, C.POINTER(Tuning)]),
+            "ua_get_tuning": (C.c_int, [vp, C.POINTER(Tuning)]),
+            "ua_set_tuning": (C.c_int, [vp, C.POINTER(Tuning)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -306,6 +319,25 @@ class Context:
         n, ms, rb = C.c_int64(), C.c_double(), C.c_int32()
         self._check(lib().ua_serve_stats(self.h, C.byref(n), C.byref(ms), C.byref(rb)))
         return {"launches": n.value, "kernel_ms": ms.value, "round_budget": rb.value}
+
+    # ---- autotuning
+    def autotune(self, queries) -> dict:
+        """Measures candidate settings on a sample of queries and keeps the best per class."""
+        arr = (Query * len(queries))(*queries)
+        t = Tuning()
+        self._check(lib().ua_autotune(self.h, arr, len(queries), C.byref(t)))
+        return t.as_dict()
+
+    def get_tuning(self) -> dict:
+        t = Tuning()
+        self._check(lib().ua_get_tuning(self.h, C.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, d: dict):
+        t = Tuning()
+        for k, v in d.items():
+            setattr(t, k, v)
+        self._check(lib().ua_set_tuning(self.h, C.byref(t)))
 
     # ---- diagnostics
     def heap_selftest(self, ops, keys, cap_nodes):

@@ -55,12 +55,26 @@ struct ua_grid {
 struct ServeEngine;
 void ua_serve_release(ua_ctx *c);  // serve.cpp
 
+namespace uart {
+// The knobs the autotuner sets (tune.cpp). A nonzero ua_config field overrides the
+// matching knob; the defaults are the untuned starting point.
+struct Tuning {
+    int grid_waves = 4;           // waves (= heaps) per batched / served grid query
+    int puzzle_waves = 64;        // waves (= heaps) per batched / served puzzle query
+    int puzzle_hash_log2[5] = {0, 0, 0, 17, 24};   // batched / served puzzles, by n
+    int plain_heaps[2] = {0, 0};  // plain whole-GPU solve, by kind (grid, puzzle); 0 = 4 per CU
+    int plain_hash_log2[5] = {0, 0, 0, 19, 24};    // plain puzzles, by n
+    bool tuned = false;
+};
+}  // namespace uart
+
 struct ua_ctx {
     int dev = 0;
     hipStream_t st = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;  // around each search launch on st
     double last_kernel_ms = 0;
     ua_config cfg{};
+    uart::Tuning tune;
     std::string err;
     int num_cu = 0;
     size_t total_mem = 0;
@@ -114,6 +128,9 @@ int puzzle_fill_desc(ua_ctx *c, int n, const uint8_t *tiles, int heaps, int hash
 int ensure_pdb(ua_ctx *c, int n);
 void fill_result(const QueryDesc &q, const QueryResult &r, ua_result *o);
 int team_bpg(ua_ctx *c, int kind);
+int plain_heaps(ua_ctx *c, int kind);
+int batch_hash_log2(ua_ctx *c, int n);
+int plain_hash_log2(ua_ctx *c, int n);
 int run_batch(ua_ctx *c, std::vector<QueryDesc> &descs, std::vector<QueryResult> &res);
 
 }  // namespace uart

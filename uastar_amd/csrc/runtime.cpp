@@ -170,13 +170,25 @@ void fill_result(const QueryDesc &q, const QueryResult &r, ua_result *o) {
 
 // ---- plain solve (whole GPU, cooperative launch) ----------------------------------
 
-static int plain_heaps(ua_ctx *c) {
+// Heaps of the plain whole-GPU solve for a query kind (KIND_GRID / KIND_PUZZLE): the
+// configured or tuned count, rounded to whole 4-wave blocks and capped at what one
+// cooperative launch can hold.
+int plain_heaps(ua_ctx *c, int kind) {
     int per_cu = 0;
     if (teams_max_blocks(&per_cu) != hipSuccess || per_cu < 1) per_cu = 1;
-    int want = c->cfg.plain_heaps > 0 ? c->cfg.plain_heaps : 4 * c->num_cu;
+    int tuned = c->tune.plain_heaps[kind == KIND_PUZZLE ? 1 : 0];
+    int want = c->cfg.plain_heaps > 0 ? c->cfg.plain_heaps : (tuned > 0 ? tuned : 4 * c->num_cu);
     int blocks = std::max(1, (want + 3) / 4);
     blocks = std::min(blocks, per_cu * c->num_cu);
     return blocks * 4;
+}
+
+// Hash slots (log2) per puzzle query: batched / served, and plain.
+int batch_hash_log2(ua_ctx *c, int n) {
+    return c->cfg.hash_log2 > 0 ? c->cfg.hash_log2 : c->tune.puzzle_hash_log2[n == 3 ? 3 : 4];
+}
+int plain_hash_log2(ua_ctx *c, int n) {
+    return c->cfg.hash_log2 > 0 ? c->cfg.hash_log2 : c->tune.plain_hash_log2[n == 3 ? 3 : 4];
 }
 
 static int run_wide(ua_ctx *c, QueryDesc &q, const Plan &p) {
@@ -341,7 +353,7 @@ extern "C" void ua_grid_free(ua_ctx *c, ua_grid *g) {
 extern "C" int ua_grid_solve(ua_ctx *c, const ua_grid *g, int sx, int sy, int ex, int ey, ua_result *res) {
     if (!c || !res) return fail(c, UA_ERR_INVALID, "null argument");
     c->last_valid = false;
-    int heaps = plain_heaps(c);
+    int heaps = plain_heaps(c, KIND_GRID);
     for (int scale = 1; scale <= 64; scale *= 4) {
         QueryDesc q;
         Plan p;
@@ -406,8 +418,8 @@ extern "C" int ua_puzzle_solve(ua_ctx *c, int n, const uint8_t *tiles, ua_result
     }
     int rc = ensure_pdb(c, n);
     if (rc) return rc;
-    int heaps = plain_heaps(c);
-    int hl = c->cfg.hash_log2 > 0 ? c->cfg.hash_log2 : (n == 3 ? 19 : 24);
+    int heaps = plain_heaps(c, KIND_PUZZLE);
+    int hl = plain_hash_log2(c, n);
     for (int tries = 0; tries < 4; ++tries, hl += 2) {
         QueryDesc q;
         Plan p;
@@ -457,8 +469,8 @@ namespace uart {
 // bound and run best on small teams; puzzles are frontier-wide and shallow and run
 // best on wide teams.
 int team_bpg(ua_ctx *c, int kind) {
-    int w = kind == KIND_PUZZLE ? (c->cfg.puzzle_team_waves > 0 ? c->cfg.puzzle_team_waves : 64)
-                                : (c->cfg.team_waves > 0 ? c->cfg.team_waves : 4);
+    int w = kind == KIND_PUZZLE ? (c->cfg.puzzle_team_waves > 0 ? c->cfg.puzzle_team_waves : c->tune.puzzle_waves)
+                                : (c->cfg.team_waves > 0 ? c->cfg.team_waves : c->tune.grid_waves);
     return std::min(64, std::max(1, (w + 3) / 4));
 }
 
@@ -552,9 +564,8 @@ extern "C" int ua_batch_solve(ua_ctx *c, const ua_query *qs, int64_t nq, ua_resu
             if (!board_ok(u.n, u.tiles)) rc = fail(c, UA_ERR_INVALID, "query %lld: bad board", (long long)i);
             else if (!board_solvable(u.n, u.tiles)) rc = UA_NO_PATH;
             else if ((rc = ensure_pdb(c, u.n)) == UA_OK)
-                rc = puzzle_fill_desc(c, u.n, u.tiles, 4 * team_bpg(c, KIND_PUZZLE),
-                                      c->cfg.hash_log2 > 0 ? c->cfg.hash_log2 : (u.n == 3 ? 17 : 24), descs[i],
-                                      plans[i]);
+                rc = puzzle_fill_desc(c, u.n, u.tiles, 4 * team_bpg(c, KIND_PUZZLE), batch_hash_log2(c, u.n),
+                                      descs[i], plans[i]);
         } else {
             rc = fail(c, UA_ERR_INVALID, "query %lld: bad kind", (long long)i);
         }

@@ -124,7 +124,7 @@ int plan_job(ua_ctx *c, ServeEngine *e, Job &j, bool *fits) {
     } else {
         rc = ensure_pdb(c, j.q.n);
         if (!rc) {
-            int base = c->cfg.hash_log2 > 0 ? c->cfg.hash_log2 : (j.q.n == 3 ? 17 : 24);
+            int base = batch_hash_log2(c, j.q.n);
             int grow = 0;
             for (int s = j.scale; s > 1; s /= 4) grow += 2;
             rc = puzzle_fill_desc(c, j.q.n, j.q.tiles, heaps, base + grow, j.d, j.plan);
