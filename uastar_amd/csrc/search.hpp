@@ -108,6 +108,34 @@ struct GroupTeam {
     }
 };
 
+// A team of the waves of one block (the one-block groups of batched and served
+// queries). Every query datum the team touches -- best-cost table or hash, heaps,
+// round slots, insert buffer -- is touched by this block alone for the whole launch, so
+// all of it is exchanged at workgroup scope: relaxed atomics resolve in this XCD's L2
+// and the barrier needs no L2 write-back or invalidation. (Between launches the kernel
+// boundary makes the state visible wherever a resumed query lands.)
+struct BlockTeam {
+    static constexpr int SLOT_SCOPE = __HIP_MEMORY_SCOPE_WORKGROUP;
+    static constexpr int DATA_SCOPE = __HIP_MEMORY_SCOPE_WORKGROUP;
+    int tw, W;
+    TeamMem m;
+    __device__ explicit BlockTeam(const TeamMem &mem) : m(mem) {
+        tw = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+        W = (int)(blockDim.x >> 6);
+    }
+    __device__ bool leader() const { return threadIdx.x == 0; }
+    template <class Hook>
+    __device__ bool sync(Hook hook) const {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __syncthreads();
+        if (threadIdx.x == 0) hook();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        return true;
+    }
+};
+
 template <int SC>
 __device__ __forceinline__ uint64_t aload64(const uint64_t *p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, SC);
@@ -161,6 +189,10 @@ struct GridProb {
                            uint32_t &err, int64_t &n_exp, int64_t &n_gen, int64_t &n_ins) const {
         uint64_t cell = it & cmask;
         int x = (int)(cell >> q.ybits), y = (int)(cell & ymask);
+        if (x >= q.H || y >= q.W) {  // never produced by a correct round: fail the query, touch nothing
+            err |= FLAG_ABORT;
+            return;
+        }
         uint64_t id = (uint64_t)x * q.W + y;
         uint64_t gk = aload64<SC>(&q.best[id]) >> 4;
         uint64_t fk = gk + octile_key(x, y, q.ex, q.ey);
@@ -302,6 +334,10 @@ struct PuzzleProb {
     __device__ void expand(uint64_t it, uint64_t (&s)[MAXS], int &cnt, uint64_t &goalmin,
                            uint32_t &err, int64_t &n_exp, int64_t &n_gen, int64_t &n_ins) const {
         uint64_t slot = it & ((1ull << 48) - 1);
+        if (slot > hmask) {  // never produced by a correct round: fail the query, touch nothing
+            err |= FLAG_ABORT;
+            return;
+        }
         uint32_t gi = 255u - (uint32_t)((it >> 48) & 255u);
         uint64_t b = aload64<SC>(&q.best[slot]);
         uint32_t v = aload32<SC>(&q.vals[slot]);
@@ -442,14 +478,22 @@ __device__ void team_solve(const QueryDesc &q, QueryResult *res, const Team &tea
             status = ST_ABORT;
             break;
         }
-        // ---- phase B: termination test, then insert this wave's share
-        uint64_t lb = __hip_atomic_load(&cur->lb, __ATOMIC_RELAXED, SS);
-        uint64_t g = __hip_atomic_load(&cur->goal, __ATOMIC_RELAXED, SS);
-        uint32_t cnt = __hip_atomic_load(&cur->cnt, __ATOMIC_RELAXED, SS);
-        uint32_t rerr = __hip_atomic_load(&cur->err, __ATOMIC_RELAXED, SS);
+        // ---- phase B: termination test, then insert this wave's share. One lane reads
+        // the round slot and hands it to the block through LDS, so every wave of the
+        // block takes its decisions from the same values.
+        __shared__ RoundSlot view;
+        if (threadIdx.x == 0) {
+            view.lb = __hip_atomic_load(&cur->lb, __ATOMIC_RELAXED, SS);
+            view.goal = __hip_atomic_load(&cur->goal, __ATOMIC_RELAXED, SS);
+            view.cnt = __hip_atomic_load(&cur->cnt, __ATOMIC_RELAXED, SS);
+            view.err = __hip_atomic_load(&cur->err, __ATOMIC_RELAXED, SS);
+        }
+        __syncthreads();
+        const uint64_t lb = view.lb, g = view.goal;
+        const uint32_t cnt = view.cnt, rerr = view.err;  // rewritten only after the next team barrier
         best_goal = best_goal < g ? best_goal : g;
         if (rerr) {
-            status = (rerr & FLAG_CAPACITY) ? ST_CAPACITY : ST_RANGE;
+            status = (rerr & FLAG_ABORT) ? ST_ABORT : (rerr & FLAG_CAPACITY) ? ST_CAPACITY : ST_RANGE;
             break;
         }
         if (lb == SENT) {  // every heap was empty: the open list is exhausted
@@ -538,6 +582,32 @@ __device__ void dispatch(const QueryDesc &q, QueryResult *res, const uint8_t *pd
 
 __device__ __forceinline__ int64_t now_ticks() { return (int64_t)__builtin_amdgcn_s_memrealtime(); }
 
+// The query loop of one group: the group leader takes the next query index of its
+// class (the hook runs on one thread inside the group barrier) and the group solves it
+// as one team, until the class's queries are exhausted.
+template <class Team>
+__device__ void group_loop(const QueryDesc *qs, QueryResult *rs, const Team &team, TeamVars *tv, uint32_t *head,
+                           int nq, int qoff, const uint8_t *pdb3, const uint8_t *pdb4) {
+    for (;;) {
+        int64_t t0 = now_ticks();
+        if (!team.sync([tv, head] {
+                uint32_t qi = atomicAdd(head, 1u);
+                __hip_atomic_store(&tv->next_q, qi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }))
+            break;
+        int k = __builtin_amdgcn_readfirstlane(
+            (int)__hip_atomic_load(&tv->next_q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        if (k >= nq) break;
+        int qi = qoff + k;
+        QueryDesc q = qs[qi];
+        dispatch(q, rs + qi, pdb3, pdb4, team);
+        if (team.leader()) {
+            rs[qi].t_start = t0;
+            rs[qi].t_end = now_ticks();
+        }
+    }
+}
+
 // Body of k_teams for one block: find this block's class and group from the launch
 // shape; the group leader takes the next query index of its class and hands it to
 // the group through the group's TeamVars; the group solves it as one team; repeat
@@ -561,26 +631,14 @@ __device__ inline void group_block(const QueryDesc *qs, QueryResult *rs, const L
     m.stats = &tv->n_pop;
     m.seg = segs + seg_off;
     m.segcap = segcap;
-    GroupTeam team(m, tv, bpg, rel % bpg);
     uint32_t *head = qhead + cls;
     const int nq = shape.nq[cls], qoff = shape.qoff[cls];
-    for (;;) {
-        int64_t t0 = now_ticks();
-        if (!team.sync([tv, head] {
-                uint32_t qi = atomicAdd(head, 1u);
-                __hip_atomic_store(&tv->next_q, qi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            }))
-            break;
-        int k = __builtin_amdgcn_readfirstlane(
-            (int)__hip_atomic_load(&tv->next_q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-        if (k >= nq) break;
-        int qi = qoff + k;
-        QueryDesc q = qs[qi];
-        dispatch(q, rs + qi, pdb3, pdb4, team);
-        if (team.leader()) {
-            rs[qi].t_start = t0;
-            rs[qi].t_end = now_ticks();
-        }
+    if (bpg == 1) {
+        BlockTeam team(m);
+        group_loop(qs, rs, team, tv, head, nq, qoff, pdb3, pdb4);
+    } else {
+        GroupTeam team(m, tv, bpg, rel % bpg);
+        group_loop(qs, rs, team, tv, head, nq, qoff, pdb3, pdb4);
     }
 }
 
