@@ -1,6 +1,7 @@
 """Benchmark of the GA* solve path on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload plain-grid1000]
+    python bench.py --copy-calib 1024     # one 1 GiB device-to-device copy (counter calibration)
 
 Prints ONE JSON line (see the keys below). A "step" is one query of the workload's
 stream; `value` is whole-job queries/s with every input already resident in HBM when
@@ -28,6 +29,37 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_teams_serve_traffic.md")
+
+
+def measured_traffic():
+    """HBM bytes per serving launch from the committed rocprofv3 counter summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), or None."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def copy_calib(mib):
+    """One device-to-device copy of `mib` MiB through the HIP runtime's copy kernel, run
+    under rocprofv3 counters to calibrate FETCH_SIZE / WRITE_SIZE against known bytes."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(n)) == 0
+    for _ in range(2):
+        assert hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    return {"copy_bytes": n, "copies": 2}
 
 
 def grid_bytes_alg(r):
@@ -244,6 +276,19 @@ def run_serve_mixed(args):
     materialize(ctx, specs)
     warm, specs = specs[: args.warmup], specs[args.warmup:]
     ctx.puzzle_build_pdb(4)
+    tuning = None
+    if not args.no_tune:
+        # the autotuner measures on its own sample (another seed of the same stream)
+        tspecs = mixed_stream(24, seed=77, max_side=4096)
+        materialize(ctx, tspecs)
+        tq = [ctx.grid_query(s["grid"], *s["s"], *s["g"]) if s["kind"] == "grid" else ctx.puzzle_query(4, s["board"])
+              for s in tspecs]
+        t = time.perf_counter()
+        tuning = ctx.autotune(tq)
+        say(f"autotune ({time.perf_counter() - t:.1f} s): {tuning}")
+        for s in tspecs:
+            if s["kind"] == "grid":
+                s["grid"].free()
     say(f"warm-up: {len(warm)} plain + served queries")
     plain_pass(ctx, warm)
     for s in warm:
@@ -268,8 +313,9 @@ def run_serve_mixed(args):
     mismatch = sum(answer(a, s) != answer(b, s) for a, b, s in zip(pres, sres, specs))
     errors = sum(r["status"] < 0 for r in pres + sres)
     kms = st1["kernel_ms"] - st0["kernel_ms"]
+    launches = st1["launches"] - st0["launches"]
     alg = sum(alg_bytes(r, s) for r, s in zip(sres, specs))
-    achieved = alg / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
+    achieved = alg / (kms * 1e-3) / 1e9 if kms > 0 else 0.0  # = per-launch bytes / average launch time
     out = {
         "metric": "queries/s + p50/p99 latency, batched serving vs plain GA*, 1 MI355X, exact",
         "value": serve_qps,
@@ -286,7 +332,8 @@ def run_serve_mixed(args):
         "config": {"workload": "serving stream: 70% grids (side log-uniform in [512, 8192], Bernoulli(p) "
                                "obstacles with p ~ U[0.2,0.4], random open start/goal), 30% random solvable "
                                "15-puzzles with the 7-8 PDB; Poisson arrivals at load x plain capacity",
-                   "mode": "serve (continuous batching, one 4-wave team per query)", "load": args.load,
+                   "mode": "serve (continuous batching; team width per class from the autotuner)",
+                   "load": args.load,
                    "arrival_rate_qps": rate},
         "p50_ms": pct(slat, 50),
         "p99_ms": pct(slat, 99),
@@ -295,9 +342,12 @@ def run_serve_mixed(args):
         "speedup_vs_plain": serve_qps / plain_qps,
         "exact_mismatches_serve_vs_plain": mismatch,
         "errors": errors,
-        "serve_launches": st1["launches"] - st0["launches"],
+        "serve_launches": launches,
+        "tuning": tuning,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_teams (serve launches)",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
+                     "kernel": "k_teams<2> (serving step)", "launches": launches,
+                     "kernel_ms_avg": kms / max(1, launches), "alg_bytes_per_launch": alg / max(1, launches),
                      "kernel_ms_total": kms, "alg_bytes_total": alg},
     }
     def diag(res, lat=None):
@@ -339,7 +389,13 @@ def main():
     ap.add_argument("--workload", default="serve-mixed", choices=["serve-mixed", "plain-grid1000"])
     ap.add_argument("--load", type=float, default=20.0, help="arrival rate as a multiple of plain capacity")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default settings)")
+    ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
+                    help="only run a device copy of MIB MiB (rocprofv3 counter calibration)")
     args = ap.parse_args()
+    if args.copy_calib:
+        print(json.dumps(copy_calib(args.copy_calib)))
+        return
     if args.gpus != 1:
         print("bench.py runs on one GPU: every query is solved on a single MI355X "
               "(multi-GPU sharding is a design note, not measured).", file=sys.stderr)
@@ -384,7 +440,7 @@ def main():
         "expanded_per_s": sum(r["expanded"] for r in res) / wall,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_wide", "kernel_ms_avg": float(np.mean(kms)),
+                     "kernel": "k_teams<0> (plain solve)", "kernel_ms_avg": float(np.mean(kms)),
                      "alg_bytes_avg": float(np.mean(alg))},
     }
     say(f"GPU: {qps:.2f} queries/s, p50 {out['p50_ms']:.2f} ms, {len(bad)} errors")

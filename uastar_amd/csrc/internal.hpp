@@ -131,6 +131,7 @@ int team_bpg(ua_ctx *c, int kind);
 int plain_heaps(ua_ctx *c, int kind);
 int batch_hash_log2(ua_ctx *c, int n);
 int plain_hash_log2(ua_ctx *c, int n);
-int run_batch(ua_ctx *c, std::vector<QueryDesc> &descs, std::vector<QueryResult> &res);
+// mode: 1 = a ua_batch_solve sub-batch, 2 = a serving step (names the kernel in profiles)
+int run_batch(ua_ctx *c, std::vector<QueryDesc> &descs, std::vector<QueryResult> &res, int mode);
 
 }  // namespace uart

@@ -9,9 +9,11 @@ hipError_t launch_grid_gen(uint64_t *bm, int H, int W, int wpr, uint64_t seed, u
                            int sy, int ex, int ey, hipStream_t st);
 hipError_t launch_grid_pack(const uint8_t *bytes, uint64_t *bm, int H, int W, int wpr, hipStream_t st);
 // The search kernel (k_teams): shape.groups[c] * shape.bpg[c] blocks of 256 threads, cooperative.
+// mode names the caller in profiles: 0 plain solve, 1 batch, 2 serving step.
 hipError_t teams_max_blocks(int *per_cu);
 hipError_t launch_teams(const QueryDesc *qs, QueryResult *rs, const LaunchShape &shape, uint32_t *qhead,
-                        TeamVars *tvs, uint64_t *segs, const uint8_t *pdb3, const uint8_t *pdb4, hipStream_t st);
+                        TeamVars *tvs, uint64_t *segs, const uint8_t *pdb3, const uint8_t *pdb4, int mode,
+                        hipStream_t st);
 hipError_t launch_grid_path(const uint64_t *best, int H, int W, int sx, int sy, int ex, int ey,
                             int32_t *ids, int64_t cap, int64_t *len, hipStream_t st);
 hipError_t launch_puzzle_moves(int n, const uint64_t *keys, const uint32_t *vals, int hash_log2,

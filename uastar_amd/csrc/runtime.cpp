@@ -224,7 +224,7 @@ static int run_wide(ua_ctx *c, QueryDesc &q, const Plan &p) {
     shape.bpg[1] = 1;
     HIPCHK(c, launch_teams(c->qd.as<QueryDesc>(), c->qr.as<QueryResult>(), shape, c->qh.as<uint32_t>(),
                            c->tv.as<TeamVars>(), c->seg.as<uint64_t>(), c->pdb[3].as<uint8_t>(),
-                           c->pdb[4].as<uint8_t>(), c->st));
+                           c->pdb[4].as<uint8_t>(), 0, c->st));
     HIPCHK(c, hipEventRecord(c->ev1, c->st));
     QueryResult r{};
     HIPCHK(c, hipMemcpyAsync(&r, c->qr.p, sizeof r, hipMemcpyDeviceToHost, c->st));
@@ -477,7 +477,7 @@ int team_bpg(ua_ctx *c, int kind) {
 // Runs the planned queries in one cooperative launch of k_teams: class 0 = puzzles on
 // wide groups, class 1 = grids on one-block groups (each desc's `heaps` was planned
 // for its class). Results come back in the order of descs.
-int run_batch(ua_ctx *c, std::vector<QueryDesc> &descs, std::vector<QueryResult> &res) {
+int run_batch(ua_ctx *c, std::vector<QueryDesc> &descs, std::vector<QueryResult> &res, int mode) {
     const int nq = (int)descs.size();
     res.assign(nq, QueryResult{});
     if (!nq) return UA_OK;
@@ -530,7 +530,7 @@ int run_batch(ua_ctx *c, std::vector<QueryDesc> &descs, std::vector<QueryResult>
     HIPCHK(c, hipEventRecord(c->ev0, c->st));
     HIPCHK(c, launch_teams(c->b_qd.as<QueryDesc>(), c->b_qr.as<QueryResult>(), shape, c->b_head.as<uint32_t>(),
                            c->b_tv.as<TeamVars>(), c->b_seg.as<uint64_t>(), c->pdb[3].as<uint8_t>(),
-                           c->pdb[4].as<uint8_t>(), c->st));
+                           c->pdb[4].as<uint8_t>(), mode, c->st));
     HIPCHK(c, hipEventRecord(c->ev1, c->st));
     std::vector<QueryResult> sres(nq);
     HIPCHK(c, hipMemcpyAsync(sres.data(), c->b_qr.p, sizeof(QueryResult) * nq, hipMemcpyDeviceToHost, c->st));
@@ -650,7 +650,7 @@ extern "C" int ua_batch_solve(ua_ctx *c, const ua_query *qs, int64_t nq, ua_resu
             HIPCHK(c, hipMemsetAsync(base, 0xFF, ff_end, c->st));
             if (zero_end > ff_end) HIPCHK(c, hipMemsetAsync(base + ff_end, 0, zero_end - ff_end, c->st));
             std::vector<QueryResult> sr;
-            int rc = run_batch(c, sub, sr);
+            int rc = run_batch(c, sub, sr, 1);
             if (rc) return rc;
             for (size_t k = pos; k < end; ++k) {
                 int64_t i = todo[k];

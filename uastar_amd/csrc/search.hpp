@@ -31,34 +31,84 @@ namespace ua {
 
 // ------------------------------------------------------------------ teams
 // A team owns: the two round slots, the two halves of the insert buffer (segcap
-// keys each) and four statistics counters. SLOT_SCOPE is the scope of the
-// round-state exchange between the team's waves; DATA_SCOPE the scope of the
-// best-cost table / hash table that several teams' kernels may touch.
+// keys each) and four statistics counters.
 struct TeamMem {
     RoundSlot *slot;   // [2]
+    RoundSlot *view;   // LDS: the round slot as read by one lane, for the whole block
     uint64_t *seg;     // [2][segcap]
     int segcap;
     uint64_t *stats;   // [4]: pops, expanded, generated, inserted
 };
 
+// Relaxed atomics at a scope chosen at run time. A one-block team is the only user of
+// its query's data -- best-cost table or hash, heaps, round slots, insert buffer --
+// for the whole launch, so it exchanges all of it at workgroup scope (atomics resolve
+// in this XCD's L2; barriers need no L2 write-back or invalidation). A team of several
+// blocks uses agent scope. Between launches the kernel boundary makes the state visible
+// wherever a resumed query lands. The flag is wave-uniform: each operation is one
+// scalar branch, and one copy of the search serves both kinds of team (two inlined
+// copies would double the kernel's register use and halve its occupancy).
+struct Scope {
+    bool wg;
+    template <class T>
+    __device__ __forceinline__ T load(const T *p) const {
+        return wg ? __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
+                  : __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    template <class T>
+    __device__ __forceinline__ void store(T *p, T v) const {
+        if (wg) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        else __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    template <class T>
+    __device__ __forceinline__ T fmin(T *p, T v) const {
+        return wg ? __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
+                  : __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    template <class T>
+    __device__ __forceinline__ T fadd(T *p, T v) const {
+        return wg ? __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
+                  : __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    template <class T>
+    __device__ __forceinline__ void fold_or(T *p, T v) const {
+        if (wg) __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        else __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    // compare-and-swap; on failure `expect` receives the current value
+    __device__ __forceinline__ bool cas(uint64_t *p, uint64_t &expect, uint64_t v) const {
+        return wg ? __hip_atomic_compare_exchange_strong(p, &expect, v, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_WORKGROUP)
+                  : __hip_atomic_compare_exchange_strong(p, &expect, v, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __device__ __forceinline__ void release() const {
+        if (wg) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        else __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    }
+    __device__ __forceinline__ void acquire() const {
+        if (wg) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        else __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+};
+
 // A group of `bpg` consecutive blocks of a cooperative launch. Round state and the
 // insert buffer live in global memory and cross a generation barrier over the
-// group's blocks with agent-scope release/acquire fences. The plain solve is one
-// group of every block; a batch is many groups (one block each by default), each
-// pulling queries from a shared counter.
+// group's blocks. The plain solve is one group of every block; a batch is many groups
+// (one block each for grids), each pulling queries from a shared counter.
 struct GroupTeam {
-    static constexpr int SLOT_SCOPE = __HIP_MEMORY_SCOPE_AGENT;
-    static constexpr int DATA_SCOPE = __HIP_MEMORY_SCOPE_AGENT;
     static constexpr uint32_t SPIN_LIMIT = 1u << 22;  // a few seconds; then the group aborts
     int tw, W;      // wave index within the group (wave-uniform), waves in the group
     int gblock;     // block index within the group
     int bpg;        // blocks per group
+    Scope sc;       // workgroup scope for one-block groups, agent scope otherwise
     TeamMem m;
     TeamVars *tv;
     __device__ GroupTeam(const TeamMem &mem, TeamVars *t, int blocks_per_group, int block_in_group)
         : m(mem), tv(t) {
         bpg = blocks_per_group;
         gblock = block_in_group;
+        sc.wg = bpg == 1;
         int wpb = (int)(blockDim.x >> 6);
         tw = __builtin_amdgcn_readfirstlane(gblock * wpb + (int)(threadIdx.x >> 6));
         W = bpg * wpb;
@@ -67,18 +117,19 @@ struct GroupTeam {
     // Generation barrier. Every block's thread 0 arrives once; the last one runs the
     // hook, resets the counter and bumps the generation. A bounded spin turns a lost
     // block into an abort that every block of the group observes, never a hang.
+    // A one-block group is one workgroup barrier around the hook.
     template <class Hook>
     __device__ bool sync(Hook hook) const {
-        constexpr int SC = SLOT_SCOPE;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        sc.release();
         __syncthreads();
-        if (bpg == 1) {  // a one-block group needs no arrival counter: same fences, no atomics
+        if (bpg == 1) {
             if (threadIdx.x == 0) hook();
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            sc.release();
             __syncthreads();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            sc.acquire();
             return true;
         }
+        constexpr int SC = __HIP_MEMORY_SCOPE_AGENT;
         __shared__ uint32_t s_ok;
         if (threadIdx.x == 0) {
             uint32_t ok = 1;
@@ -108,51 +159,6 @@ struct GroupTeam {
     }
 };
 
-// A team of the waves of one block (the one-block groups of batched and served
-// queries). Every query datum the team touches -- best-cost table or hash, heaps,
-// round slots, insert buffer -- is touched by this block alone for the whole launch, so
-// all of it is exchanged at workgroup scope: relaxed atomics resolve in this XCD's L2
-// and the barrier needs no L2 write-back or invalidation. (Between launches the kernel
-// boundary makes the state visible wherever a resumed query lands.)
-struct BlockTeam {
-    static constexpr int SLOT_SCOPE = __HIP_MEMORY_SCOPE_WORKGROUP;
-    static constexpr int DATA_SCOPE = __HIP_MEMORY_SCOPE_WORKGROUP;
-    int tw, W;
-    TeamMem m;
-    __device__ explicit BlockTeam(const TeamMem &mem) : m(mem) {
-        tw = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-        W = (int)(blockDim.x >> 6);
-    }
-    __device__ bool leader() const { return threadIdx.x == 0; }
-    template <class Hook>
-    __device__ bool sync(Hook hook) const {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        __syncthreads();
-        if (threadIdx.x == 0) hook();
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        __syncthreads();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-        return true;
-    }
-};
-
-template <int SC>
-__device__ __forceinline__ uint64_t aload64(const uint64_t *p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, SC);
-}
-template <int SC>
-__device__ __forceinline__ uint32_t aload32(const uint32_t *p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, SC);
-}
-template <int SC>
-__device__ __forceinline__ uint64_t amin64(uint64_t *p, uint64_t v) {
-    return __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, SC);
-}
-template <int SC>
-__device__ __forceinline__ uint32_t amin32(uint32_t *p, uint32_t v) {
-    return __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, SC);
-}
-
 // ------------------------------------------------------------------ grid problem
 // Item key: (fq << cbits) | (x << ybits | y), fq = f_key >> qshift.
 // Best word per cell: (g_key << 4) | dir, dir = move into the cell (8 = start).
@@ -174,18 +180,18 @@ struct GridProb {
     }
     __device__ bool trivial() const { return q.sx == q.ex && q.sy == q.ey; }
     // Called by one lane: writes the start record and returns its item key.
-    template <int SC>
-    __device__ uint64_t seed() const {
+    __device__ uint64_t seed(const Scope &sc) const {
         uint64_t id = (uint64_t)q.sx * q.W + q.sy;
-        __hip_atomic_store(&q.best[id], (uint64_t)8, __ATOMIC_RELAXED, SC);
+        sc.store(&q.best[id], (uint64_t)8);
         return item(octile_key(q.sx, q.sy, q.ex, q.ey), q.sx, q.sy);
     }
     __device__ bool goal_done(uint64_t goal, uint64_t lb) const {
         return goal != SENT && goal <= ((lb >> q.cbits) << q.qshift);
     }
     // Per-lane expansion of one popped item; successors land in s[0..7] (SENT = none).
-    template <int SC>
-    __device__ void expand(uint64_t it, uint64_t (&s)[MAXS], int &cnt, uint64_t &goalmin,
+    // Memory is touched in two waves of independent accesses: the cell's best word and
+    // the 3x3 bitmap window around it (six words), then one atomicMin per legal move.
+    __device__ __forceinline__ void expand(const Scope &sc, uint64_t it, uint64_t (&s)[MAXS], int &cnt, uint64_t &goalmin,
                            uint32_t &err, int64_t &n_exp, int64_t &n_gen, int64_t &n_ins) const {
         uint64_t cell = it & cmask;
         int x = (int)(cell >> q.ybits), y = (int)(cell & ymask);
@@ -194,32 +200,61 @@ struct GridProb {
             return;
         }
         uint64_t id = (uint64_t)x * q.W + y;
-        uint64_t gk = aload64<SC>(&q.best[id]) >> 4;
+        // issue every load first
+        uint64_t bw = sc.load(&q.best[id]);
+        const int clo = y > 0 ? y - 1 : 0, chi = y + 1 < q.W ? y + 1 : q.W - 1;
+        uint64_t wlo[3], whi[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            int r = x + i - 1;
+            r = r < 0 ? 0 : (r >= q.H ? q.H - 1 : r);
+            const uint64_t *row = q.bitmap + (size_t)r * q.wpr;
+            wlo[i] = row[clo >> 6];
+            whi[i] = row[chi >> 6];
+        }
+        uint64_t gk = bw >> 4;
         uint64_t fk = gk + octile_key(x, y, q.ex, q.ey);
         if ((fk >> q.qshift) != (it >> q.cbits)) return;  // stale: a better entry exists
         ++n_exp;
-        if (!open(x, y)) return;                          // source-cell mask rule
+        // window bit (3 * i + j) = cell (x + i - 1, y + j - 1) is in range and open
+        uint32_t win = 0;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            int r = x + i - 1;
+            if (r < 0 || r >= q.H) continue;
+            uint64_t wc = (y & 63) == 0 ? whi[i] : wlo[i];  // the word of column y
+            if (y > 0 && ((wlo[i] >> ((y - 1) & 63)) & 1ull)) win |= 1u << (3 * i);
+            if ((wc >> (y & 63)) & 1ull) win |= 1u << (3 * i + 1);
+            if (y + 1 < q.W && ((whi[i] >> ((y + 1) & 63)) & 1ull)) win |= 1u << (3 * i + 2);
+        }
+        if (!((win >> 4) & 1u)) return;  // source-cell mask rule
+        uint64_t old[8];
+        uint32_t live = 0;
 #pragma unroll
         for (int d = 0; d < 8; ++d) {
             int nx = x + kDX[d], ny = y + kDY[d];
             if (nx < 0 || nx >= q.H || ny < 0 || ny >= q.W) continue;
-            uint64_t nid = (uint64_t)nx * q.W + ny;
-            bool is_goal = nid == goal_id;
-            if (!is_goal && !open(nx, ny)) continue;        // blocked non-goal: a dead end
+            bool is_goal = (uint64_t)nx * q.W + ny == goal_id;
+            if (!is_goal && !((win >> (3 * (kDX[d] + 1) + (kDY[d] + 1))) & 1u)) continue;  // blocked: a dead end
             ++n_gen;
             uint64_t ng = gk + (d < 4 ? KEY_ONE : KEY_SQRT2);
-            uint64_t nf = ng + octile_key(nx, ny, q.ex, q.ey);
-            if (nf >= KEY_LIMIT) {
+            if (ng + octile_key(nx, ny, q.ex, q.ey) >= KEY_LIMIT) {
                 err |= FLAG_RANGE;
                 continue;
             }
-            uint64_t old = amin64<SC>(&q.best[nid], (ng << 4) | (uint64_t)d);
-            if ((old >> 4) <= ng) continue;
+            live |= 1u << d;
+            old[d] = sc.fmin(&q.best[(uint64_t)nx * q.W + ny], (ng << 4) | (uint64_t)d);
+        }
+#pragma unroll
+        for (int d = 0; d < 8; ++d) {
+            uint64_t ng = gk + (d < 4 ? KEY_ONE : KEY_SQRT2);
+            if (!((live >> d) & 1u) || (old[d] >> 4) <= ng) continue;
             ++n_ins;
-            if (is_goal) {
+            int nx = x + kDX[d], ny = y + kDY[d];
+            if ((uint64_t)nx * q.W + ny == goal_id) {
                 goalmin = goalmin < ng ? goalmin : ng;
             } else {
-                s[d] = item(nf, nx, ny);
+                s[d] = item(ng + octile_key(nx, ny, q.ex, q.ey), nx, ny);
                 ++cnt;
             }
         }
@@ -277,94 +312,135 @@ __device__ __forceinline__ uint32_t pattern_rank(uint64_t s, int p) {
     return code;
 }
 
-template <int N>
-__device__ __forceinline__ int puzzle_h(uint64_t s, const uint8_t *pdb) {
-    using S = PSpec<N>;
+// The partitions of PSpec<3> / PSpec<4> as one table indexed at run time, so that a
+// single copy of the search serves both board sizes.
+struct PTab {
+    int np;             // partitions
+    int k[2];           // tiles per partition
+    int t[2][8];        // the tiles
+    uint32_t m[2][8];   // rank radices
+    uint64_t off[2];    // byte offset of each partition's table in the PDB
+};
+__constant__ static const PTab kPTab[2] = {
+    {PSpec<3>::NP, {PSpec<3>::K[0], 0},
+     {{PSpec<3>::T[0][0], PSpec<3>::T[0][1], PSpec<3>::T[0][2], PSpec<3>::T[0][3], PSpec<3>::T[0][4],
+       PSpec<3>::T[0][5], PSpec<3>::T[0][6], PSpec<3>::T[0][7]},
+      {0, 0, 0, 0, 0, 0, 0, 0}},
+     {{PSpec<3>::M[0][0], PSpec<3>::M[0][1], PSpec<3>::M[0][2], PSpec<3>::M[0][3], PSpec<3>::M[0][4],
+       PSpec<3>::M[0][5], PSpec<3>::M[0][6], PSpec<3>::M[0][7]},
+      {0, 0, 0, 0, 0, 0, 0, 0}},
+     {PSpec<3>::OFF[0], 0}},
+    {PSpec<4>::NP, {PSpec<4>::K[0], PSpec<4>::K[1]},
+     {{PSpec<4>::T[0][0], PSpec<4>::T[0][1], PSpec<4>::T[0][2], PSpec<4>::T[0][3], PSpec<4>::T[0][4],
+       PSpec<4>::T[0][5], PSpec<4>::T[0][6], PSpec<4>::T[0][7]},
+      {PSpec<4>::T[1][0], PSpec<4>::T[1][1], PSpec<4>::T[1][2], PSpec<4>::T[1][3], PSpec<4>::T[1][4],
+       PSpec<4>::T[1][5], PSpec<4>::T[1][6], PSpec<4>::T[1][7]}},
+     {{PSpec<4>::M[0][0], PSpec<4>::M[0][1], PSpec<4>::M[0][2], PSpec<4>::M[0][3], PSpec<4>::M[0][4],
+       PSpec<4>::M[0][5], PSpec<4>::M[0][6], PSpec<4>::M[0][7]},
+      {PSpec<4>::M[1][0], PSpec<4>::M[1][1], PSpec<4>::M[1][2], PSpec<4>::M[1][3], PSpec<4>::M[1][4],
+       PSpec<4>::M[1][5], PSpec<4>::M[1][6], PSpec<4>::M[1][7]}},
+     {PSpec<4>::OFF[0], PSpec<4>::OFF[1]}},
+};
+
+__device__ __forceinline__ uint32_t pattern_rank_rt(const PTab &tb, uint64_t s, int p) {
+    uint32_t mask = 0, code = 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        if (j < tb.k[p]) {
+            int pos = nib_pos(s, tb.t[p][j]);
+            uint32_t v = (uint32_t)pos - __builtin_popcount(mask & ((1u << pos) - 1u));
+            code += v * tb.m[p][j];
+            mask |= 1u << pos;
+        }
+    }
+    return code;
+}
+
+__device__ __forceinline__ int puzzle_h(const PTab &tb, uint64_t s, const uint8_t *pdb) {
     int h = 0;
 #pragma unroll
-    for (int p = 0; p < S::NP; ++p) h += pdb[S::OFF[p] + pattern_rank<N>(s, p)];
+    for (int p = 0; p < 2; ++p)
+        if (p < tb.np) h += pdb[tb.off[p] + pattern_rank_rt(tb, s, p)];
     return h;
 }
 
 // Item key: (f << 56) | ((255 - g) << 48) | slot  (ties: larger g first).
 // Hash: keys[slot] = board (0 = empty), vals[slot] = (g << 4) | dir (15 = start).
-template <int N>
+// The side n (3 or 4) is a run-time value.
 struct PuzzleProb {
     static constexpr int MAXS = 4;
     static constexpr int MAXPROBE = 256;
     QueryDesc q;
     const uint8_t *pdb;
+    const PTab &tb;
+    int n;
     uint64_t hmask;
     uint64_t goal_b;
-    __device__ PuzzleProb(const QueryDesc &qd, const uint8_t *p) : q(qd), pdb(p) {
+    __device__ PuzzleProb(const QueryDesc &qd, const uint8_t *p) : q(qd), pdb(p), tb(kPTab[qd.n == 3 ? 0 : 1]) {
+        n = qd.n == 3 ? 3 : 4;
         hmask = (1ull << q.hash_log2) - 1;
-        goal_b = goal_board(N);
+        goal_b = goal_board(n);
     }
     __device__ bool trivial() const { return q.start_state == goal_b; }
-    template <int SC>
-    __device__ uint64_t find_or_insert(uint64_t b, uint32_t &err) const {
+    __device__ __forceinline__ uint64_t find_or_insert(const Scope &sc, uint64_t b, uint32_t &err) const {
         uint64_t slot = mix64(b) & hmask;
         for (int i = 0; i < MAXPROBE; ++i) {
-            uint64_t k = aload64<SC>(&q.best[slot]);
+            uint64_t k = sc.load(&q.best[slot]);
             if (k == b) return slot;
             if (k == 0) {
                 uint64_t exp = 0;
-                if (__hip_atomic_compare_exchange_strong(&q.best[slot], &exp, b, __ATOMIC_RELAXED,
-                                                         __ATOMIC_RELAXED, SC) ||
-                    exp == b)
-                    return slot;
+                if (sc.cas(&q.best[slot], exp, b) || exp == b) return slot;
             }
             slot = (slot + 1) & hmask;
         }
         err |= FLAG_CAPACITY;
         return SENT;
     }
-    template <int SC>
-    __device__ uint64_t seed() const {
+    __device__ uint64_t seed(const Scope &sc) const {
         uint32_t err = 0;
-        uint64_t slot = find_or_insert<SC>(q.start_state, err);
-        __hip_atomic_store(&q.vals[slot], (uint32_t)15, __ATOMIC_RELAXED, SC);
-        uint64_t h = (uint64_t)puzzle_h<N>(q.start_state, pdb);
+        uint64_t slot = find_or_insert(sc, q.start_state, err);
+        sc.store(&q.vals[slot], (uint32_t)15);
+        uint64_t h = (uint64_t)puzzle_h(tb, q.start_state, pdb);
         return (h << 56) | (255ull << 48) | slot;
     }
     __device__ bool goal_done(uint64_t goal, uint64_t lb) const {
         return goal != SENT && (goal >> 32) <= (lb >> 56);
     }
-    template <int SC>
-    __device__ void expand(uint64_t it, uint64_t (&s)[MAXS], int &cnt, uint64_t &goalmin,
-                           uint32_t &err, int64_t &n_exp, int64_t &n_gen, int64_t &n_ins) const {
+    __device__ __forceinline__ void expand(const Scope &sc, uint64_t it, uint64_t (&s)[MAXS], int &cnt,
+                                           uint64_t &goalmin, uint32_t &err, int64_t &n_exp, int64_t &n_gen,
+                                           int64_t &n_ins) const {
         uint64_t slot = it & ((1ull << 48) - 1);
         if (slot > hmask) {  // never produced by a correct round: fail the query, touch nothing
             err |= FLAG_ABORT;
             return;
         }
         uint32_t gi = 255u - (uint32_t)((it >> 48) & 255u);
-        uint64_t b = aload64<SC>(&q.best[slot]);
-        uint32_t v = aload32<SC>(&q.vals[slot]);
+        uint64_t b = sc.load(&q.best[slot]);
+        uint32_t v = sc.load(&q.vals[slot]);
         uint32_t gb = v >> 4;
         if (gb != gi) return;  // stale
         ++n_exp;
         int dprev = (int)(v & 15u);
         int bl = nib_pos(b, 0);
-        int bx = bl / N, by = bl % N;
+        int bx = n == 4 ? bl >> 2 : bl / 3, by = bl - bx * n;
 #pragma unroll
         for (int d = 0; d < 4; ++d) {
             if (dprev < 4 && d == (dprev ^ 1)) continue;  // the move back to the parent
             int nx = bx + kDX[d], ny = by + kDY[d];
-            if (nx < 0 || nx >= N || ny < 0 || ny >= N) continue;
+            if (nx < 0 || nx >= n || ny < 0 || ny >= n) continue;
             ++n_gen;
-            int nb = nx * N + ny;
+            int nb = nx * n + ny;
             uint64_t t = (b >> (4 * nb)) & 15ull;
             uint64_t ns = b + (t << (4 * bl)) - (t << (4 * nb));
             uint32_t ng = gb + 1;
-            int h = puzzle_h<N>(ns, pdb);
+            int h = puzzle_h(tb, ns, pdb);
             if (ng + h > 255u) {
                 err |= FLAG_RANGE;
                 continue;
             }
-            uint64_t ns_slot = find_or_insert<SC>(ns, err);
+            uint64_t ns_slot = find_or_insert(sc, ns, err);
             if (ns_slot == SENT) continue;
-            uint32_t old = amin32<SC>(&q.vals[ns_slot], (ng << 4) | (uint32_t)d);
+            uint32_t old = sc.fmin(&q.vals[ns_slot], (ng << 4) | (uint32_t)d);
             if ((old >> 4) <= ng) continue;
             ++n_ins;
             if (ns == goal_b) {
@@ -381,8 +457,7 @@ struct PuzzleProb {
 // ------------------------------------------------------------------ the team search
 template <class Team, class Prob>
 __device__ void team_solve(const QueryDesc &q, QueryResult *res, const Team &team, const Prob &P) {
-    constexpr int SS = Team::SLOT_SCOPE;
-    constexpr int DS = Team::DATA_SCOPE;
+    const Scope sc = team.sc;
     const int l = lane_id();
     const int tw = team.tw, W = team.W;
     RoundSlot *const slots = team.m.slot;
@@ -408,13 +483,12 @@ __device__ void team_solve(const QueryDesc &q, QueryResult *res, const Team &tea
     uint64_t *rec_heap = q.resume ? q.resume + RESUME_HEADER + (size_t)tw * RESUME_PER_HEAP : nullptr;
     if (team.leader()) {
         for (int i = 0; i < 2; ++i) {
-            __hip_atomic_store(&slots[i].lb, SENT, __ATOMIC_RELAXED, SS);
-            __hip_atomic_store(&slots[i].goal, SENT, __ATOMIC_RELAXED, SS);
-            __hip_atomic_store(&slots[i].cnt, 0u, __ATOMIC_RELAXED, SS);
-            __hip_atomic_store(&slots[i].err, 0u, __ATOMIC_RELAXED, SS);
+            sc.store(&slots[i].lb, SENT);
+            sc.store(&slots[i].goal, SENT);
+            sc.store(&slots[i].cnt, 0u);
+            sc.store(&slots[i].err, 0u);
         }
-        for (int i = 0; i < 4; ++i)
-            __hip_atomic_store(&stats[i], resumed ? q.resume[3 + i] : (uint64_t)0, __ATOMIC_RELAXED, SS);
+        for (int i = 0; i < 4; ++i) sc.store(&stats[i], resumed ? q.resume[3 + i] : (uint64_t)0);
     }
     if (resumed) {
         heap.root = rec_heap[l];
@@ -423,7 +497,7 @@ __device__ void team_solve(const QueryDesc &q, QueryResult *res, const Team &tea
         heap.pc = (int)rec_heap[129];
     } else if (tw == 0) {
         uint64_t it = SENT;
-        if (l == 0) it = P.template seed<DS>();
+        if (l == 0) it = P.seed(sc);
         heap.part = it;
         heap.pc = 1;
     }
@@ -441,7 +515,7 @@ __device__ void team_solve(const QueryDesc &q, QueryResult *res, const Team &tea
         // ---- phase A: pop, expand, relax, append
         uint64_t it = heap.pop();
         uint64_t wmin = readlane64(it, 0);
-        if (l == 0 && wmin != SENT) __hip_atomic_fetch_min(&cur->lb, wmin, __ATOMIC_RELAXED, SS);
+        if (l == 0 && wmin != SENT) sc.fmin(&cur->lb, wmin);
 
         uint64_t s[Prob::MAXS];
 #pragma unroll
@@ -451,28 +525,28 @@ __device__ void team_solve(const QueryDesc &q, QueryResult *res, const Team &tea
         uint32_t err = 0;
         if (it != SENT) {
             ++n_pop;
-            P.template expand<DS>(it, s, c, goalmin, err, n_exp, n_gen, n_ins);
+            P.expand(sc, it, s, c, goalmin, err, n_exp, n_gen, n_ins);
         }
         int total;
         int off = wave_excl_scan(c, &total);
         uint32_t base = 0;
-        if (l == 0 && total) base = __hip_atomic_fetch_add(&cur->cnt, (uint32_t)total, __ATOMIC_RELAXED, SS);
+        if (l == 0 && total) base = sc.fadd(&cur->cnt, (uint32_t)total);
         base = __shfl(base, 0);
         uint64_t *out = cur_seg + base + off;
 #pragma unroll
         for (int i = 0; i < Prob::MAXS; ++i)
             if (s[i] != SENT) *out++ = s[i];
         goalmin = wave_min64(goalmin);
-        if (l == 0 && goalmin != SENT) __hip_atomic_fetch_min(&cur->goal, goalmin, __ATOMIC_RELAXED, SS);
+        if (l == 0 && goalmin != SENT) sc.fmin(&cur->goal, goalmin);
         for (int j = 32; j > 0; j >>= 1) err |= __shfl_xor(err, j);
-        if (l == 0 && err) __hip_atomic_fetch_or(&cur->err, err, __ATOMIC_RELAXED, SS);
+        if (l == 0 && err) sc.fold_or(&cur->err, err);
 
         RoundSlot *nxt = &slots[(r + 1) & 1];
-        bool ok = team.sync([nxt] {
-            __hip_atomic_store(&nxt->lb, SENT, __ATOMIC_RELAXED, Team::SLOT_SCOPE);
-            __hip_atomic_store(&nxt->goal, SENT, __ATOMIC_RELAXED, Team::SLOT_SCOPE);
-            __hip_atomic_store(&nxt->cnt, 0u, __ATOMIC_RELAXED, Team::SLOT_SCOPE);
-            __hip_atomic_store(&nxt->err, 0u, __ATOMIC_RELAXED, Team::SLOT_SCOPE);
+        bool ok = team.sync([nxt, sc] {
+            sc.store(&nxt->lb, SENT);
+            sc.store(&nxt->goal, SENT);
+            sc.store(&nxt->cnt, 0u);
+            sc.store(&nxt->err, 0u);
         });
         if (!ok) {
             status = ST_ABORT;
@@ -481,16 +555,16 @@ __device__ void team_solve(const QueryDesc &q, QueryResult *res, const Team &tea
         // ---- phase B: termination test, then insert this wave's share. One lane reads
         // the round slot and hands it to the block through LDS, so every wave of the
         // block takes its decisions from the same values.
-        __shared__ RoundSlot view;
+        RoundSlot *const view = team.m.view;
         if (threadIdx.x == 0) {
-            view.lb = __hip_atomic_load(&cur->lb, __ATOMIC_RELAXED, SS);
-            view.goal = __hip_atomic_load(&cur->goal, __ATOMIC_RELAXED, SS);
-            view.cnt = __hip_atomic_load(&cur->cnt, __ATOMIC_RELAXED, SS);
-            view.err = __hip_atomic_load(&cur->err, __ATOMIC_RELAXED, SS);
+            view->lb = sc.load(&cur->lb);
+            view->goal = sc.load(&cur->goal);
+            view->cnt = sc.load(&cur->cnt);
+            view->err = sc.load(&cur->err);
         }
         __syncthreads();
-        const uint64_t lb = view.lb, g = view.goal;
-        const uint32_t cnt = view.cnt, rerr = view.err;  // rewritten only after the next team barrier
+        const uint64_t lb = view->lb, g = view->goal;
+        const uint32_t cnt = view->cnt, rerr = view->err;  // rewritten only after the next team barrier
         best_goal = best_goal < g ? best_goal : g;
         if (rerr) {
             status = (rerr & FLAG_ABORT) ? ST_ABORT : (rerr & FLAG_CAPACITY) ? ST_CAPACITY : ST_RANGE;
@@ -519,7 +593,7 @@ __device__ void team_solve(const QueryDesc &q, QueryResult *res, const Team &tea
             v = wave_sort(v);
             fits &= heap.insert(v, sreal);
         }
-        if (!fits && l == 0) __hip_atomic_fetch_or(&nxt->err, FLAG_CAPACITY, __ATOMIC_RELAXED, SS);
+        if (!fits && l == 0) sc.fold_or(&nxt->err, (uint32_t)FLAG_CAPACITY);
         if (q.round_budget > 0 && r + 1 - r0 >= q.round_budget && q.resume != nullptr) {
             status = ST_SUSPENDED;  // every wave reaches this with the same r
             break;
@@ -540,27 +614,27 @@ __device__ void team_solve(const QueryDesc &q, QueryResult *res, const Team &tea
     n_gen = wave_sum64(n_gen);
     n_ins = wave_sum64(n_ins);
     if (l == 0) {
-        __hip_atomic_fetch_add(&stats[0], (uint64_t)n_pop, __ATOMIC_RELAXED, SS);
-        __hip_atomic_fetch_add(&stats[1], (uint64_t)n_exp, __ATOMIC_RELAXED, SS);
-        __hip_atomic_fetch_add(&stats[2], (uint64_t)n_gen, __ATOMIC_RELAXED, SS);
-        __hip_atomic_fetch_add(&stats[3], (uint64_t)n_ins, __ATOMIC_RELAXED, SS);
+        sc.fadd(&stats[0], (uint64_t)n_pop);
+        sc.fadd(&stats[1], (uint64_t)n_exp);
+        sc.fadd(&stats[2], (uint64_t)n_gen);
+        sc.fadd(&stats[3], (uint64_t)n_ins);
     }
     team.sync([] {});
     if (team.leader()) {
         if (status == ST_SUSPENDED) {
             q.resume[1] = (uint64_t)(r + 1);
             q.resume[2] = best_goal;
-            for (int i = 0; i < 4; ++i) q.resume[3 + i] = __hip_atomic_load(&stats[i], __ATOMIC_RELAXED, SS);
+            for (int i = 0; i < 4; ++i) q.resume[3 + i] = sc.load(&stats[i]);
             q.resume[0] = 1;
         }
         res->status = status;
         res->found = found;
         res->goal = best_goal;
         res->rounds = r + 1;
-        res->pops = (int64_t)__hip_atomic_load(&stats[0], __ATOMIC_RELAXED, SS);
-        res->expanded = (int64_t)__hip_atomic_load(&stats[1], __ATOMIC_RELAXED, SS);
-        res->generated = (int64_t)__hip_atomic_load(&stats[2], __ATOMIC_RELAXED, SS);
-        res->inserted = (int64_t)__hip_atomic_load(&stats[3], __ATOMIC_RELAXED, SS) + 1;
+        res->pops = (int64_t)sc.load(&stats[0]);
+        res->expanded = (int64_t)sc.load(&stats[1]);
+        res->generated = (int64_t)sc.load(&stats[2]);
+        res->inserted = (int64_t)sc.load(&stats[3]) + 1;
         res->flags = 0;
     }
 }
@@ -571,11 +645,8 @@ __device__ void dispatch(const QueryDesc &q, QueryResult *res, const uint8_t *pd
     if (q.kind == KIND_GRID) {
         GridProb P(q);
         team_solve(q, res, team, P);
-    } else if (q.n == 4) {
-        PuzzleProb<4> P(q, pdb4);
-        team_solve(q, res, team, P);
     } else {
-        PuzzleProb<3> P(q, pdb3);
+        PuzzleProb P(q, q.n == 3 ? pdb3 : pdb4);
         team_solve(q, res, team, P);
     }
 }
@@ -626,20 +697,17 @@ __device__ inline void group_block(const QueryDesc *qs, QueryResult *rs, const L
     size_t seg_off = cls ? (size_t)shape.groups[0] * 2 * shape.segcap[0] + (size_t)(rel / bpg) * 2 * segcap
                          : (size_t)(rel / bpg) * 2 * segcap;
     TeamVars *tv = tvs + group;
+    __shared__ RoundSlot view;
     TeamMem m;
     m.slot = tv->slot;
+    m.view = &view;
     m.stats = &tv->n_pop;
     m.seg = segs + seg_off;
     m.segcap = segcap;
     uint32_t *head = qhead + cls;
     const int nq = shape.nq[cls], qoff = shape.qoff[cls];
-    if (bpg == 1) {
-        BlockTeam team(m);
-        group_loop(qs, rs, team, tv, head, nq, qoff, pdb3, pdb4);
-    } else {
-        GroupTeam team(m, tv, bpg, rel % bpg);
-        group_loop(qs, rs, team, tv, head, nq, qoff, pdb3, pdb4);
-    }
+    GroupTeam team(m, tv, bpg, rel % bpg);
+    group_loop(qs, rs, team, tv, head, nq, qoff, pdb3, pdb4);
 }
 
 }  // namespace ua

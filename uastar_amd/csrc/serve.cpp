@@ -253,7 +253,7 @@ extern "C" int ua_serve_step(ua_ctx *c, int64_t *n_active, int64_t *n_pending) {
             descs.push_back(j.d);
         }
         std::vector<QueryResult> res;
-        int rc = run_batch(c, descs, res);
+        int rc = run_batch(c, descs, res, 2);
         if (rc) return rc;
         double launch_ms = c->last_kernel_ms;
         e->launches++;

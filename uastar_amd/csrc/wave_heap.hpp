@@ -177,17 +177,26 @@ struct WaveHeap {
     }
 
     // Inserts a full batch F whose keys are all >= max(root), as a new leaf,
-    // merging top-down along the path (the root is skipped). Returns false on overflow.
+    // merging top-down along the path (the root is skipped). The path's nodes are
+    // loaded PF at a time before any merge, so a deep heap costs few load latencies.
+    // Returns false on overflow.
     __device__ bool push_full(uint64_t F) {
+        constexpr int PF = 4;
         int target = n + 1;
         if (target > cap) return false;
         int depth = 31 - __builtin_clz((unsigned)target);
-        for (int L = 1; L < depth; ++L) {
-            int v = target >> (depth - L);
-            uint64_t X = load(v);
-            if (readlane64(X, 63) <= readlane64(F, 0)) continue;
-            wave_merge(X, F);
-            store(v, X);
+        for (int L0 = 1; L0 < depth; L0 += PF) {
+            uint64_t X[PF];
+#pragma unroll
+            for (int j = 0; j < PF; ++j)
+                if (L0 + j < depth) X[j] = load(target >> (depth - L0 - j));
+#pragma unroll
+            for (int j = 0; j < PF; ++j) {
+                if (L0 + j >= depth) break;
+                if (readlane64(X[j], 63) <= readlane64(F, 0)) continue;
+                wave_merge(X[j], F);
+                store(target >> (depth - L0 - j), X[j]);
+            }
         }
         store(target, F);
         n = target;
