@@ -366,6 +366,11 @@ def run_serve_mixed(args):
                             "expanded_p50": pct([x["expanded"] for x in rr], 50)}
         return rows, by
     srows, out["serve_diag"] = diag(sres, slat)
+    done_at = [a + l for a, l in zip(arrivals, slat)]
+    out["serve_tail"] = [dict(kind=specs[k]["kind"], size=specs[k].get("H", 0), arrival_ms=round(arrivals[k], 1),
+                              done_ms=round(done_at[k], 1), rounds=sres[k]["rounds"],
+                              device_ms=round(sres[k]["device_ms"], 1))
+                         for k in sorted(range(len(specs)), key=lambda k: -done_at[k])[:5]]
     _, out["plain_diag"] = diag(pres)
     for x in sorted(srows, key=lambda x: -(x["lat_ms"] or 0))[:8]:
         say(f"slowest served: {x}")

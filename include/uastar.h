@@ -42,12 +42,12 @@ typedef struct ua_ctx ua_ctx;
 typedef struct ua_grid ua_grid; /* a device-resident grid bitmap */
 
 typedef struct {
-    int32_t team_waves;        /* waves (= heaps) per grid query in batch/serve; 0 = 4 */
-    int32_t plain_heaps;       /* heaps (= waves) of the plain whole-GPU solve; 0 = 4 per CU */
+    int32_t team_waves;        /* waves (= heaps) per grid query in batch/serve (multiple of 16); 0 = tuned */
+    int32_t plain_heaps;       /* heaps (= waves) of the plain whole-GPU solve; 0 = tuned (4 per CU) */
     int32_t max_rounds;        /* 0 = 1<<26 */
-    int32_t hash_log2;         /* puzzle hash table size per query; 0 = autotuned */
+    int32_t hash_log2;         /* puzzle hash table size per query; 0 = tuned */
     uint64_t mem_budget;       /* device bytes the batch/serve planner may use; 0 = 80-85% of free */
-    int32_t puzzle_team_waves; /* waves (= heaps) per puzzle query in batch/serve; 0 = 64 */
+    int32_t puzzle_team_waves; /* waves (= heaps) per puzzle query in batch/serve (multiple of 16); 0 = tuned */
     int32_t reserved;
 } ua_config;
 
@@ -127,8 +127,9 @@ int ua_serve_step(ua_ctx *ctx, int64_t *n_active, int64_t *n_pending);
 int ua_serve_poll(ua_ctx *ctx, uint64_t *tickets, ua_result *results, int64_t cap, int64_t *n_out);
 /* steps until nothing is pending or active */
 int ua_serve_drain(ua_ctx *ctx);
-/* launches so far, their summed duration (HIP events) and the current round budget */
-int ua_serve_stats(ua_ctx *ctx, int64_t *launches, double *kernel_ms, int32_t *round_budget);
+/* launches so far, their summed duration (HIP events) and the device-time budget of one
+ * launch in microseconds */
+int ua_serve_stats(ua_ctx *ctx, int64_t *launches, double *kernel_ms, int32_t *budget_us);
 
 /* ---- autotuning (no reference counterpart: uAstar fixes its launch shape and table
  * sizes at compile time, pathway/GPU-solver.hpp:5-17 and puzzle/GPU-solver.cuh:12-22).
@@ -138,16 +139,18 @@ int ua_serve_stats(ua_ctx *ctx, int64_t *launches, double *kernel_ms, int32_t *r
  * heap count of the plain whole-GPU solve. Nonzero ua_config fields are left alone.
  * The sample is solved several times; the state of an earlier ua_batch_solve is freed. */
 typedef struct {
-    int32_t grid_team_waves;    /* waves (= heaps) per batched / served grid query */
-    int32_t puzzle_team_waves;  /* waves (= heaps) per batched / served 15-puzzle query */
-    int32_t puzzle_hash_log2;   /* hash slots (log2) per batched / served 15-puzzle query */
-    int32_t plain_grid_heaps;   /* heaps of the plain whole-GPU grid solve */
-    int32_t plain_puzzle_heaps; /* heaps of the plain whole-GPU 15-puzzle solve */
-    int32_t plain_hash_log2;    /* hash slots (log2) of a plain 15-puzzle solve */
-    int32_t trials;             /* settings measured by the last ua_autotune */
-    int32_t reserved;
-    double est_grid_ms;         /* estimated stream time of the sample's grids at the chosen width */
-    double est_puzzle_ms;       /* the same for its 15-puzzles */
+    int32_t grid_team_waves;       /* waves per batched / served grid below grid_large_cells (16, 32, ...) */
+    int32_t grid_large_team_waves; /* waves per batched / served grid of at least grid_large_cells cells */
+    int64_t grid_large_cells;      /* size threshold between the two grid classes */
+    int32_t puzzle_team_waves;     /* waves (= heaps) per batched / served 15-puzzle query */
+    int32_t puzzle_hash_log2;      /* hash slots (log2) per batched / served 15-puzzle query */
+    int32_t plain_grid_heaps;      /* heaps of the plain whole-GPU grid solve */
+    int32_t plain_puzzle_heaps;    /* heaps of the plain whole-GPU 15-puzzle solve */
+    int32_t plain_hash_log2;       /* hash slots (log2) of a plain 15-puzzle solve */
+    int32_t trials;                /* settings measured by the last ua_autotune */
+    double est_grid_ms;            /* estimated stream time of the sample's small grids at the chosen width */
+    double est_grid_large_ms;      /* ... of its large grids */
+    double est_puzzle_ms;          /* ... of its 15-puzzles */
 } ua_tuning;
 int ua_autotune(ua_ctx *ctx, const ua_query *sample, int64_t n, ua_tuning *out);
 /* the settings in effect (tuned, configured or default) */

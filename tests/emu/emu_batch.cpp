@@ -1,5 +1,5 @@
 // emu_batch.cpp -- runs the device batch loop (ua::group_block from search.hpp) for one
-// block (one group, 4 waves) on host threads. Built by tests/test_emu_search.py.
+// one-block team of 4, 8 or 16 waves on host threads. Built by tests/test_emu_search.py.
 #include "hip_emu.hpp"
 
 #include <vector>
@@ -8,16 +8,16 @@
 
 extern "C" int emu_batch(const ua::QueryDesc *qs, ua::QueryResult *rs, int nq, int waves, const uint8_t *pdb3,
                          const uint8_t *pdb4) {
-    if (waves != 4) return -1;  // k_teams blocks are 4 waves; one block = one group here
+    if (waves != 4 && waves != 8 && waves != 16) return -1;
     const int segcap = waves * 64 * 8;
     std::vector<uint64_t> segs((size_t)2 * segcap, 0);
     ua::TeamVars tv{};
-    uint32_t qhead[2] = {0, 0};
+    uint32_t qhead[ua::kClasses] = {0, 0, 0};
     ua::LaunchShape shape{};
-    shape.groups[1] = 1;  // one one-block group of class 1 takes every query
-    shape.bpg[0] = shape.bpg[1] = 1;
-    shape.nq[1] = nq;
-    shape.segcap[1] = segcap;
+    shape.groups[ua::CLS_GRID] = 1;  // one one-block group takes every query
+    for (int k = 0; k < ua::kClasses; ++k) shape.bpg[k] = 1;
+    shape.nq[ua::CLS_GRID] = nq;
+    shape.segcap[ua::CLS_GRID] = segcap;
     std::barrier<> bar(waves * 64);
     std::vector<emu::Wave> ws(waves);
     emu::Block blk{&bar, ws.data(), (unsigned)(waves * 64)};

@@ -99,9 +99,10 @@ class Batch:
         return list(res)
 
 
-def test_emulated_batch_grids(emu, oracle):
+@pytest.mark.parametrize("waves", [4, 16])
+def test_emulated_batch_grids(emu, oracle, waves):
     rng = np.random.default_rng(42)
-    b = Batch(waves=4)
+    b = Batch(waves=waves)
     exp = []
     for i in range(6):
         H, W = int(rng.integers(2, 60)), int(rng.integers(2, 60))

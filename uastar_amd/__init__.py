@@ -56,13 +56,14 @@ class Query(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("grid_team_waves", C.c_int32), ("puzzle_team_waves", C.c_int32), ("puzzle_hash_log2", C.c_int32),
+    _fields_ = [("grid_team_waves", C.c_int32), ("grid_large_team_waves", C.c_int32),
+                ("grid_large_cells", C.c_int64), ("puzzle_team_waves", C.c_int32), ("puzzle_hash_log2", C.c_int32),
                 ("plain_grid_heaps", C.c_int32), ("plain_puzzle_heaps", C.c_int32), ("plain_hash_log2", C.c_int32),
-                ("trials", C.c_int32), ("reserved", C.c_int32), ("est_grid_ms", C.c_double),
+                ("trials", C.c_int32), ("est_grid_ms", C.c_double), ("est_grid_large_ms", C.c_double),
                 ("est_puzzle_ms", C.c_double)]
 
     def as_dict(self):
-        return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
+        return {f: getattr(self, f) for f, _ in self._fields_}
 
 
 _lib = None
@@ -318,7 +319,7 @@ class Context:
     def serve_stats(self):
         n, ms, rb = C.c_int64(), C.c_double(), C.c_int32()
         self._check(lib().ua_serve_stats(self.h, C.byref(n), C.byref(ms), C.byref(rb)))
-        return {"launches": n.value, "kernel_ms": ms.value, "round_budget": rb.value}
+        return {"launches": n.value, "kernel_ms": ms.value, "budget_us": rb.value}
 
     # ---- autotuning
     def autotune(self, queries) -> dict:

@@ -83,7 +83,8 @@ enum : int32_t {
     ST_SUSPENDED = 2,   // round budget of a serving launch used up; state saved for resumption
 };
 // error bits raised by the kernels
-enum : uint32_t { FLAG_CAPACITY = 1, FLAG_RANGE = 2, FLAG_ABORT = 4 };
+// FLAG_TIMEUP is no error: the launch's time budget ran out (serving suspends the query)
+enum : uint32_t { FLAG_CAPACITY = 1, FLAG_RANGE = 2, FLAG_ABORT = 4, FLAG_TIMEUP = 8 };
 
 // One search instance as the kernels see it. All pointers are device pointers.
 struct QueryDesc {
@@ -103,21 +104,38 @@ struct QueryDesc {
     uint64_t *heap;          // heap node storage, heaps * (heap_cap-1) * 64 u64
     uint64_t start_state;    // puzzle
     uint64_t *resume;        // serving: resume record (resume_words(heaps) u64), or null
-    int32_t round_budget;    // serving: rounds per launch before suspending (0 = no limit)
+    int32_t round_budget;    // serving: at most this many rounds per launch (0 = no limit)
     int32_t pad2;
 };
 
-// Shape of one k_teams launch: two classes of groups. Class c has groups[c] groups
-// of bpg[c] blocks (class 0 first in block order); its groups pull the queries
-// qs[qoff[c] .. qoff[c] + nq[c]) from counter qhead[c]. A group's insert buffer
-// holds segcap[c] keys per round half.
+// The search kernel runs blocks of 16 wavefronts (1024 threads). A team is a group of
+// whole blocks: 16, 32, 64 ... waves (= heaps).
+constexpr int kBlockWaves = 16;
+
+// Instance classes of a batched / served launch, each with its own team width.
+constexpr int kClasses = 3;
+enum : int { CLS_PUZZLE = 0, CLS_GRID = 1, CLS_GRID_LARGE = 2 };
+
+// Shape of one k_teams launch. Class c has groups[c] groups of bpg[c] blocks (classes
+// in order of c in block order); its groups pull the queries qs[qoff[c] .. qoff[c] +
+// nq[c]) from counter qhead[c]. A group's insert buffer holds segcap[c] keys per
+// round half.
 struct LaunchShape {
-    int32_t groups[2];
-    int32_t bpg[2];
-    int32_t nq[2];
-    int32_t qoff[2];
-    int32_t segcap[2];
+    int32_t groups[kClasses];
+    int32_t bpg[kClasses];
+    int32_t nq[kClasses];
+    int32_t qoff[kClasses];
+    int32_t segcap[kClasses];
+    int32_t pad;
+    // serving: each group stops starting queries, and suspends the one it runs, once it
+    // has been busy this long in the launch (s_memrealtime ticks, 100 MHz; 0 = no limit)
+    int64_t budget_ticks;
 };
+UA_HD inline int shape_blocks(const LaunchShape &s) {
+    int b = 0;
+    for (int c = 0; c < kClasses; ++c) b += s.groups[c] * s.bpg[c];
+    return b;
+}
 
 // Resume record of a suspended query: header, then per heap its root batch,
 // partial batch and (n, pc). Written when a serving launch's round budget runs out.
@@ -153,6 +171,7 @@ struct TeamVars {
     uint32_t abort;
     uint32_t next_q;     // query index handed from the group leader to the group
     uint64_t n_pop, n_exp, n_gen, n_ins;
+    uint64_t deadline;   // serving: device time at which the group's budget ends (0 = unset)
 };
 
 }  // namespace ua

@@ -59,7 +59,9 @@ namespace uart {
 // The knobs the autotuner sets (tune.cpp). A nonzero ua_config field overrides the
 // matching knob; the defaults are the untuned starting point.
 struct Tuning {
-    int grid_waves = 4;           // waves (= heaps) per batched / served grid query
+    int grid_waves = 16;          // waves (= heaps) per batched / served grid query below grid_large_cells
+    int grid_large_waves = 16;    // ... and at or above grid_large_cells
+    int64_t grid_large_cells = 1ll << 22;
     int puzzle_waves = 64;        // waves (= heaps) per batched / served puzzle query
     int puzzle_hash_log2[5] = {0, 0, 0, 17, 24};   // batched / served puzzles, by n
     int plain_heaps[2] = {0, 0};  // plain whole-GPU solve, by kind (grid, puzzle); 0 = 4 per CU
@@ -127,11 +129,17 @@ uint64_t pack_board(int n, const uint8_t *t);
 int puzzle_fill_desc(ua_ctx *c, int n, const uint8_t *tiles, int heaps, int hash_log2, QueryDesc &q, Plan &p);
 int ensure_pdb(ua_ctx *c, int n);
 void fill_result(const QueryDesc &q, const QueryResult &r, ua_result *o);
-int team_bpg(ua_ctx *c, int kind);
+// waves per team (a multiple of kBlockWaves) of a batched / served query of this kind
+// and size, and its instance class (CLS_*)
+int team_waves(ua_ctx *c, int kind, int64_t cells);
+int class_of(ua_ctx *c, int kind, int64_t cells);
 int plain_heaps(ua_ctx *c, int kind);
 int batch_hash_log2(ua_ctx *c, int n);
 int plain_hash_log2(ua_ctx *c, int n);
-// mode: 1 = a ua_batch_solve sub-batch, 2 = a serving step (names the kernel in profiles)
-int run_batch(ua_ctx *c, std::vector<QueryDesc> &descs, std::vector<QueryResult> &res, int mode);
+// mode: 1 = a ua_batch_solve sub-batch, 2 = a serving step (names the kernel in profiles).
+// budget_ticks > 0: each group stops after that much device time (serving); `init`, if
+// given, is the result reported for a query the launch does not reach.
+int run_batch(ua_ctx *c, std::vector<QueryDesc> &descs, std::vector<QueryResult> &res, int mode,
+              int64_t budget_ticks = 0, const std::vector<QueryResult> *init = nullptr);
 
 }  // namespace uart

@@ -9,15 +9,16 @@
 
 namespace ua {
 
-// The search kernel: a cooperative launch of 4-wave blocks split into groups (see
-// LaunchShape); each group is a team that solves queries of its class from a counter.
-// Plain solve: one group of every block and one query. Batch / serve: wide groups for
-// puzzles, one-block groups for grids. MODE only names the caller (0 plain, 1 batch,
-// 2 serve), so that profiles report the three uses as separate kernels.
+// The search kernel: a cooperative launch of 16-wave blocks (1024 threads) split into
+// groups (see LaunchShape); each group is a team that solves queries of its class
+// from a counter. Plain solve: one group of every block and one query. Batch / serve:
+// one-block teams for grids (workgroup scope), multi-block groups for 15-puzzles. MODE
+// only names the caller (0 plain, 1 batch, 2 serving step), so that profiles report
+// the three uses as separate kernels.
 template <int MODE>
-__global__ __launch_bounds__(256) void k_teams(const QueryDesc *qs, QueryResult *rs, LaunchShape shape,
-                                               uint32_t *qhead, TeamVars *tvs, uint64_t *segs,
-                                               const uint8_t *pdb3, const uint8_t *pdb4) {
+__global__ __launch_bounds__(1024) void k_teams(const QueryDesc *qs, QueryResult *rs, LaunchShape shape,
+                                                uint32_t *qhead, TeamVars *tvs, uint64_t *segs,
+                                                const uint8_t *pdb3, const uint8_t *pdb4) {
     group_block(qs, rs, shape, qhead, tvs, segs, pdb3, pdb4);
 }
 
@@ -231,18 +232,18 @@ hipError_t launch_grid_pack(const uint8_t *bytes, uint64_t *bm, int H, int W, in
 }
 
 hipError_t teams_max_blocks(int *per_cu) {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_teams<0>, 256, 0);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_teams<0>, kBlockWaves * 64, 0);
 }
 
 hipError_t launch_teams(const QueryDesc *qs, QueryResult *rs, const LaunchShape &shape, uint32_t *qhead,
                         TeamVars *tvs, uint64_t *segs, const uint8_t *pdb3, const uint8_t *pdb4, int mode,
                         hipStream_t st) {
-    int blocks = shape.groups[0] * shape.bpg[0] + shape.groups[1] * shape.bpg[1];
+    int blocks = shape_blocks(shape);
     LaunchShape sh = shape;
     void *args[] = {(void *)&qs, (void *)&rs, (void *)&sh, (void *)&qhead, (void *)&tvs, (void *)&segs,
                     (void *)&pdb3, (void *)&pdb4};
     const void *fn = mode == 2 ? (const void *)k_teams<2> : mode == 1 ? (const void *)k_teams<1> : (const void *)k_teams<0>;
-    return hipLaunchCooperativeKernel(fn, dim3(blocks), dim3(256), args, 0, st);
+    return hipLaunchCooperativeKernel(fn, dim3(blocks), dim3(kBlockWaves * 64), args, 0, st);
 }
 
 hipError_t launch_grid_path(const uint64_t *best, int H, int W, int sx, int sy, int ex, int ey,

@@ -8,8 +8,8 @@ namespace ua {
 hipError_t launch_grid_gen(uint64_t *bm, int H, int W, int wpr, uint64_t seed, uint64_t thr, int sx,
                            int sy, int ex, int ey, hipStream_t st);
 hipError_t launch_grid_pack(const uint8_t *bytes, uint64_t *bm, int H, int W, int wpr, hipStream_t st);
-// The search kernel (k_teams): shape.groups[c] * shape.bpg[c] blocks of 256 threads, cooperative.
-// mode names the caller in profiles: 0 plain solve, 1 batch, 2 serving step.
+// The search kernel (k_teams): shape_blocks(shape) blocks of kBlockWaves wavefronts,
+// cooperative. mode names the caller in profiles: 0 plain solve, 1 batch, 2 serving step.
 hipError_t teams_max_blocks(int *per_cu);
 hipError_t launch_teams(const QueryDesc *qs, QueryResult *rs, const LaunchShape &shape, uint32_t *qhead,
                         TeamVars *tvs, uint64_t *segs, const uint8_t *pdb3, const uint8_t *pdb4, int mode,

@@ -178,9 +178,9 @@ int plain_heaps(ua_ctx *c, int kind) {
     if (teams_max_blocks(&per_cu) != hipSuccess || per_cu < 1) per_cu = 1;
     int tuned = c->tune.plain_heaps[kind == KIND_PUZZLE ? 1 : 0];
     int want = c->cfg.plain_heaps > 0 ? c->cfg.plain_heaps : (tuned > 0 ? tuned : 4 * c->num_cu);
-    int blocks = std::max(1, (want + 3) / 4);
+    int blocks = std::max(1, (want + kBlockWaves - 1) / kBlockWaves);
     blocks = std::min(blocks, per_cu * c->num_cu);
-    return blocks * 4;
+    return blocks * kBlockWaves;
 }
 
 // Hash slots (log2) per puzzle query: batched / served, and plain.
@@ -193,7 +193,7 @@ int plain_hash_log2(ua_ctx *c, int n) {
 
 static int run_wide(ua_ctx *c, QueryDesc &q, const Plan &p) {
     int heaps = q.heaps;
-    int blocks = heaps / 4;
+    int blocks = heaps / kBlockWaves;
     int segcap = heaps * 64 * (q.kind == KIND_GRID ? 8 : 4);
     HIPCHK(c, c->best.ensure(p.best_bytes));
     if (p.vals_bytes) HIPCHK(c, c->vals.ensure(p.vals_bytes));
@@ -221,7 +221,7 @@ static int run_wide(ua_ctx *c, QueryDesc &q, const Plan &p) {
     shape.bpg[0] = blocks;
     shape.nq[0] = 1;
     shape.segcap[0] = segcap;
-    shape.bpg[1] = 1;
+    for (int k = 1; k < kClasses; ++k) shape.bpg[k] = 1;
     HIPCHK(c, launch_teams(c->qd.as<QueryDesc>(), c->qr.as<QueryResult>(), shape, c->qh.as<uint32_t>(),
                            c->tv.as<TeamVars>(), c->seg.as<uint64_t>(), c->pdb[3].as<uint8_t>(),
                            c->pdb[4].as<uint8_t>(), 0, c->st));
@@ -257,9 +257,11 @@ extern "C" int ua_create(int ordinal, const ua_config *cfg, ua_ctx **out) {
     ua_ctx *c = new ua_ctx();
     c->dev = ordinal;
     if (cfg) c->cfg = *cfg;
-    if (hipSetDevice(ordinal) != hipSuccess || hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
-        delete c;
+    bool ok = hipSetDevice(ordinal) == hipSuccess &&
+              hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreate(&c->ev0) == hipSuccess && hipEventCreate(&c->ev1) == hipSuccess;
+    if (!ok) {
+        ua_destroy(c);
         return UA_ERR_HIP;
     }
     hipDeviceProp_t prop;
@@ -282,9 +284,9 @@ extern "C" void ua_destroy(ua_ctx *c) {
                       &c->pathbuf, &c->b_qd, &c->b_qr, &c->b_head, &c->b_tv, &c->b_seg})
         b->release();
     for (auto &b : c->b_arenas) b.release();
-    (void)hipEventDestroy(c->ev0);
-    (void)hipEventDestroy(c->ev1);
-    (void)hipStreamDestroy(c->st);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->st) (void)hipStreamDestroy(c->st);
     delete c;
 }
 
@@ -465,58 +467,82 @@ extern "C" int ua_puzzle_moves(ua_ctx *c, int32_t *dirs, int64_t cap, int64_t *l
 
 namespace uart {
 
-// Blocks (of 4 waves) per batched / served query team, by kind: grids are path-length
-// bound and run best on small teams; puzzles are frontier-wide and shallow and run
-// best on wide teams.
-int team_bpg(ua_ctx *c, int kind) {
-    int w = kind == KIND_PUZZLE ? (c->cfg.puzzle_team_waves > 0 ? c->cfg.puzzle_team_waves : c->tune.puzzle_waves)
-                                : (c->cfg.team_waves > 0 ? c->cfg.team_waves : c->tune.grid_waves);
-    return std::min(64, std::max(1, (w + 3) / 4));
+// Waves per team of a batched / served query (a multiple of kBlockWaves: one block
+// for grids by default, several for 15-puzzles), chosen per instance class -- puzzles,
+// grids below and at or above grid_large_cells cells -- by the configuration or the
+// autotuner.
+int class_of(ua_ctx *c, int kind, int64_t cells) {
+    if (kind == KIND_PUZZLE) return CLS_PUZZLE;
+    return cells >= c->tune.grid_large_cells ? CLS_GRID_LARGE : CLS_GRID;
 }
 
-// Runs the planned queries in one cooperative launch of k_teams: class 0 = puzzles on
-// wide groups, class 1 = grids on one-block groups (each desc's `heaps` was planned
-// for its class). Results come back in the order of descs.
-int run_batch(ua_ctx *c, std::vector<QueryDesc> &descs, std::vector<QueryResult> &res, int mode) {
+int team_waves(ua_ctx *c, int kind, int64_t cells) {
+    int w;
+    if (kind == KIND_PUZZLE) w = c->cfg.puzzle_team_waves > 0 ? c->cfg.puzzle_team_waves : c->tune.puzzle_waves;
+    else if (c->cfg.team_waves > 0) w = c->cfg.team_waves;
+    else w = class_of(c, kind, cells) == CLS_GRID_LARGE ? c->tune.grid_large_waves : c->tune.grid_waves;
+    int per_cu = 1;
+    if (teams_max_blocks(&per_cu) != hipSuccess || per_cu < 1) per_cu = 1;
+    int blocks = std::max(1, std::min((w + kBlockWaves - 1) / kBlockWaves, c->num_cu * per_cu));
+    return blocks * kBlockWaves;
+}
+
+// Runs the planned queries in one cooperative launch of k_teams. Each query's class
+// (CLS_*) gets groups of heaps / kBlockWaves blocks; the resident blocks are shared
+// among the classes in proportion to their demand (queries x blocks per team).
+// Results come back in the order of descs.
+int run_batch(ua_ctx *c, std::vector<QueryDesc> &descs, std::vector<QueryResult> &res, int mode,
+              int64_t budget_ticks, const std::vector<QueryResult> *init) {
     const int nq = (int)descs.size();
     res.assign(nq, QueryResult{});
     if (!nq) return UA_OK;
-    std::vector<int> order;
-    order.reserve(nq);
-    for (int pass = 0; pass < 2; ++pass)
-        for (int i = 0; i < nq; ++i)
-            if ((descs[i].kind == KIND_PUZZLE) == (pass == 0)) order.push_back(i);
-    int n0 = 0;
-    for (int i = 0; i < nq; ++i) n0 += descs[i].kind == KIND_PUZZLE;
-    const int n1 = nq - n0;
     int per_cu = 0;
     HIPCHK(c, teams_max_blocks(&per_cu));
     const int resident = std::max(1, c->num_cu * std::max(1, per_cu));
+    std::vector<int> order;
+    order.reserve(nq);
     LaunchShape shape{};
-    shape.bpg[0] = std::min(team_bpg(c, KIND_PUZZLE), resident);
-    shape.bpg[1] = std::min(team_bpg(c, KIND_GRID), resident);
-    for (int i = 0; i < nq; ++i) {
-        int want = 4 * (descs[i].kind == KIND_PUZZLE ? shape.bpg[0] : shape.bpg[1]);
-        if (descs[i].heaps != want) return fail(c, UA_ERR_INVALID, "query %d planned for %d heaps, launch has %d",
-                                                i, descs[i].heaps, want);
+    for (int k = 0; k < kClasses; ++k) {
+        shape.bpg[k] = 0;
+        shape.qoff[k] = (int)order.size();
+        for (int i = 0; i < nq; ++i) {
+            const QueryDesc &q = descs[i];
+            if (class_of(c, q.kind, q.kind == KIND_GRID ? (int64_t)q.H * q.W : 0) != k) continue;
+            int bpg = q.heaps / kBlockWaves;
+            if (bpg < 1 || q.heaps % kBlockWaves || (shape.bpg[k] && shape.bpg[k] != bpg) || bpg > resident)
+                return fail(c, UA_ERR_INVALID, "query %d planned for %d heaps, not a team width of its class", i,
+                            q.heaps);
+            shape.bpg[k] = bpg;
+            order.push_back(i);
+            ++shape.nq[k];
+        }
+        if (!shape.bpg[k]) shape.bpg[k] = 1;
+        shape.segcap[k] = shape.bpg[k] * kBlockWaves * 64 * 8;  // at most 8 successors per pop
     }
-    int blocks0 = 0;
-    if (n0) {
-        int share = n1 ? resident / 2 : resident;  // both classes present: split the device
-        blocks0 = std::max(shape.bpg[0], std::min(n0 * shape.bpg[0], share / shape.bpg[0] * shape.bpg[0]));
-        blocks0 = std::min(blocks0, resident / shape.bpg[0] * shape.bpg[0]);
+    double demand = 0;
+    for (int k = 0; k < kClasses; ++k) demand += (double)shape.nq[k] * shape.bpg[k];
+    int used = 0;
+    for (int k = 0; k < kClasses; ++k) {
+        if (!shape.nq[k]) continue;
+        int want = (int)(resident * shape.nq[k] * shape.bpg[k] / demand);
+        shape.groups[k] = std::max(1, std::min(shape.nq[k], want / shape.bpg[k]));
+        used += shape.groups[k] * shape.bpg[k];
     }
-    shape.groups[0] = n0 ? blocks0 / shape.bpg[0] : 0;
-    shape.groups[1] = n1 ? std::max(1, std::min(n1, (resident - blocks0) / shape.bpg[1])) : 0;
-    shape.nq[0] = n0;
-    shape.nq[1] = n1;
-    shape.qoff[0] = 0;
-    shape.qoff[1] = n0;
-    shape.segcap[0] = 4 * shape.bpg[0] * 64 * 4;  // puzzles: at most 4 successors per pop
-    shape.segcap[1] = 4 * shape.bpg[1] * 64 * 8;  // grids: at most 8
-    const int groups = shape.groups[0] + shape.groups[1];
-    const size_t seg_words =
-        (size_t)shape.groups[0] * 2 * shape.segcap[0] + (size_t)shape.groups[1] * 2 * shape.segcap[1];
+    while (used > resident) {  // rounding up to one group per class can overshoot
+        int big = 0;
+        for (int k = 1; k < kClasses; ++k)
+            if (shape.groups[k] * shape.bpg[k] > shape.groups[big] * shape.bpg[big]) big = k;
+        if (shape.groups[big] <= 1) return fail(c, UA_ERR_INVALID, "launch does not fit the device");
+        --shape.groups[big];
+        used -= shape.bpg[big];
+    }
+    int groups = 0;
+    size_t seg_words = 0;
+    for (int k = 0; k < kClasses; ++k) {
+        groups += shape.groups[k];
+        seg_words += (size_t)shape.groups[k] * 2 * shape.segcap[k];
+    }
+    shape.budget_ticks = budget_ticks;
     std::vector<QueryDesc> sorted(nq);
     for (int k = 0; k < nq; ++k) sorted[k] = descs[order[k]];
     HIPCHK(c, c->b_qd.ensure(sizeof(QueryDesc) * nq));
@@ -525,6 +551,12 @@ int run_batch(ua_ctx *c, std::vector<QueryDesc> &descs, std::vector<QueryResult>
     HIPCHK(c, c->b_tv.ensure(sizeof(TeamVars) * groups));
     HIPCHK(c, c->b_seg.ensure(seg_words * 8));
     HIPCHK(c, hipMemcpyAsync(c->b_qd.p, sorted.data(), sizeof(QueryDesc) * nq, hipMemcpyHostToDevice, c->st));
+    std::vector<QueryResult> sinit;
+    if (init) {
+        sinit.resize(nq);
+        for (int k = 0; k < nq; ++k) sinit[k] = (*init)[order[k]];
+        HIPCHK(c, hipMemcpyAsync(c->b_qr.p, sinit.data(), sizeof(QueryResult) * nq, hipMemcpyHostToDevice, c->st));
+    }
     HIPCHK(c, hipMemsetAsync(c->b_head.p, 0, 64, c->st));
     HIPCHK(c, hipMemsetAsync(c->b_tv.p, 0, sizeof(TeamVars) * groups, c->st));
     HIPCHK(c, hipEventRecord(c->ev0, c->st));
@@ -558,13 +590,14 @@ extern "C" int ua_batch_solve(ua_ctx *c, const ua_query *qs, int64_t nq, ua_resu
         const ua_query &u = qs[i];
         int rc;
         if (u.kind == UA_QUERY_GRID) {
-            rc = grid_fill_desc(c, u.grid, u.sx, u.sy, u.ex, u.ey, 4 * team_bpg(c, KIND_GRID), 1, descs[i],
+            rc = grid_fill_desc(c, u.grid, u.sx, u.sy, u.ex, u.ey,
+                                team_waves(c, KIND_GRID, u.grid ? (int64_t)u.grid->H * u.grid->W : 0), 1, descs[i],
                                 plans[i]);
         } else if (u.kind == UA_QUERY_PUZZLE) {
             if (!board_ok(u.n, u.tiles)) rc = fail(c, UA_ERR_INVALID, "query %lld: bad board", (long long)i);
             else if (!board_solvable(u.n, u.tiles)) rc = UA_NO_PATH;
             else if ((rc = ensure_pdb(c, u.n)) == UA_OK)
-                rc = puzzle_fill_desc(c, u.n, u.tiles, 4 * team_bpg(c, KIND_PUZZLE), batch_hash_log2(c, u.n),
+                rc = puzzle_fill_desc(c, u.n, u.tiles, team_waves(c, KIND_PUZZLE, 0), batch_hash_log2(c, u.n),
                                       descs[i], plans[i]);
         } else {
             rc = fail(c, UA_ERR_INVALID, "query %lld: bad kind", (long long)i);
@@ -665,11 +698,11 @@ extern "C" int ua_batch_solve(ua_ctx *c, const ua_query *qs, int64_t nq, ua_resu
             scale[i] *= 4;
             if (descs[i].kind == KIND_GRID) {
                 const ua_query &u = qs[i];
-                grid_fill_desc(c, u.grid, u.sx, u.sy, u.ex, u.ey, 4 * team_bpg(c, KIND_GRID), scale[i], descs[i],
+                grid_fill_desc(c, u.grid, u.sx, u.sy, u.ex, u.ey, descs[i].heaps, scale[i], descs[i],
                                plans[i]);
             } else {
                 int hl = descs[i].hash_log2 + 2;
-                puzzle_fill_desc(c, qs[i].n, qs[i].tiles, 4 * team_bpg(c, KIND_PUZZLE), hl, descs[i], plans[i]);
+                puzzle_fill_desc(c, qs[i].n, qs[i].tiles, descs[i].heaps, hl, descs[i], plans[i]);
             }
         }
         todo.swap(retry);

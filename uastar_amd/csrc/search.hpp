@@ -92,6 +92,8 @@ struct Scope {
     }
 };
 
+__device__ __forceinline__ int64_t now_ticks() { return (int64_t)__builtin_amdgcn_s_memrealtime(); }
+
 // A group of `bpg` consecutive blocks of a cooperative launch. Round state and the
 // insert buffer live in global memory and cross a generation barrier over the
 // group's blocks. The plain solve is one group of every block; a batch is many groups
@@ -542,11 +544,14 @@ __device__ void team_solve(const QueryDesc &q, QueryResult *res, const Team &tea
         if (l == 0 && err) sc.fold_or(&cur->err, err);
 
         RoundSlot *nxt = &slots[(r + 1) & 1];
-        bool ok = team.sync([nxt, sc] {
+        TeamVars *const tvp = team.tv;
+        bool ok = team.sync([nxt, cur, sc, tvp] {
             sc.store(&nxt->lb, SENT);
             sc.store(&nxt->goal, SENT);
             sc.store(&nxt->cnt, 0u);
             sc.store(&nxt->err, 0u);
+            uint64_t dl = __hip_atomic_load(&tvp->deadline, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (dl && (uint64_t)now_ticks() > dl) sc.fold_or(&cur->err, (uint32_t)FLAG_TIMEUP);
         });
         if (!ok) {
             status = ST_ABORT;
@@ -566,7 +571,7 @@ __device__ void team_solve(const QueryDesc &q, QueryResult *res, const Team &tea
         const uint64_t lb = view->lb, g = view->goal;
         const uint32_t cnt = view->cnt, rerr = view->err;  // rewritten only after the next team barrier
         best_goal = best_goal < g ? best_goal : g;
-        if (rerr) {
+        if (rerr & ~(uint32_t)FLAG_TIMEUP) {
             status = (rerr & FLAG_ABORT) ? ST_ABORT : (rerr & FLAG_CAPACITY) ? ST_CAPACITY : ST_RANGE;
             break;
         }
@@ -594,8 +599,8 @@ __device__ void team_solve(const QueryDesc &q, QueryResult *res, const Team &tea
             fits &= heap.insert(v, sreal);
         }
         if (!fits && l == 0) sc.fold_or(&nxt->err, (uint32_t)FLAG_CAPACITY);
-        if (q.round_budget > 0 && r + 1 - r0 >= q.round_budget && q.resume != nullptr) {
-            status = ST_SUSPENDED;  // every wave reaches this with the same r
+        if (q.resume != nullptr && ((rerr & FLAG_TIMEUP) || (q.round_budget > 0 && r + 1 - r0 >= q.round_budget))) {
+            status = ST_SUSPENDED;  // every wave reaches this with the same r and view
             break;
         }
     }
@@ -651,24 +656,33 @@ __device__ void dispatch(const QueryDesc &q, QueryResult *res, const uint8_t *pd
     }
 }
 
-__device__ __forceinline__ int64_t now_ticks() { return (int64_t)__builtin_amdgcn_s_memrealtime(); }
 
 // The query loop of one group: the group leader takes the next query index of its
 // class (the hook runs on one thread inside the group barrier) and the group solves it
 // as one team, until the class's queries are exhausted.
+// With a time budget the group's first hook fixes its deadline; after the deadline it
+// takes no more queries, and the round barrier of the query it runs flags FLAG_TIMEUP.
 template <class Team>
 __device__ void group_loop(const QueryDesc *qs, QueryResult *rs, const Team &team, TeamVars *tv, uint32_t *head,
-                           int nq, int qoff, const uint8_t *pdb3, const uint8_t *pdb4) {
+                           int nq, int qoff, int64_t budget, const uint8_t *pdb3, const uint8_t *pdb4) {
     for (;;) {
         int64_t t0 = now_ticks();
-        if (!team.sync([tv, head] {
-                uint32_t qi = atomicAdd(head, 1u);
-                __hip_atomic_store(&tv->next_q, qi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (!team.sync([tv, head, budget] {
+                constexpr int SC = __HIP_MEMORY_SCOPE_AGENT;
+                uint32_t qi = 0xFFFFFFFFu;  // none
+                uint64_t now = (uint64_t)now_ticks();
+                uint64_t dl = __hip_atomic_load(&tv->deadline, __ATOMIC_RELAXED, SC);
+                if (budget > 0 && dl == 0) {
+                    dl = now + (uint64_t)budget;
+                    __hip_atomic_store(&tv->deadline, dl, __ATOMIC_RELAXED, SC);
+                }
+                if (dl == 0 || now <= dl) qi = atomicAdd(head, 1u);
+                __hip_atomic_store(&tv->next_q, qi, __ATOMIC_RELAXED, SC);
             }))
             break;
-        int k = __builtin_amdgcn_readfirstlane(
+        uint32_t k = (uint32_t)__builtin_amdgcn_readfirstlane(
             (int)__hip_atomic_load(&tv->next_q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-        if (k >= nq) break;
+        if (k >= (uint32_t)nq) break;
         int qi = qoff + k;
         QueryDesc q = qs[qi];
         dispatch(q, rs + qi, pdb3, pdb4, team);
@@ -680,34 +694,32 @@ __device__ void group_loop(const QueryDesc *qs, QueryResult *rs, const Team &tea
 }
 
 // Body of k_teams for one block: find this block's class and group from the launch
-// shape; the group leader takes the next query index of its class and hands it to
-// the group through the group's TeamVars; the group solves it as one team; repeat
-// until the class's queries are exhausted. The index is wave-uniform, so query
-// descriptors are read with scalar loads.
+// shape, then run the group's query loop.
 __device__ inline void group_block(const QueryDesc *qs, QueryResult *rs, const LaunchShape &shape,
                                    uint32_t *qhead, TeamVars *tvs, uint64_t *segs, const uint8_t *pdb3,
                                    const uint8_t *pdb4) {
-    const int b = (int)blockIdx.x;
-    const int first1 = shape.groups[0] * shape.bpg[0];  // first block of class 1
-    const int cls = b < first1 ? 0 : 1;
-    const int rel = cls ? b - first1 : b;
+    int rel = (int)blockIdx.x, cls = 0, group = 0;
+    size_t seg_off = 0;
+    while (cls + 1 < kClasses && rel >= shape.groups[cls] * shape.bpg[cls]) {
+        rel -= shape.groups[cls] * shape.bpg[cls];
+        group += shape.groups[cls];
+        seg_off += (size_t)shape.groups[cls] * 2 * (size_t)shape.segcap[cls];
+        ++cls;
+    }
     const int bpg = shape.bpg[cls];
-    const int group = (cls ? shape.groups[0] : 0) + rel / bpg;
+    group += rel / bpg;
     const int segcap = shape.segcap[cls];
-    size_t seg_off = cls ? (size_t)shape.groups[0] * 2 * shape.segcap[0] + (size_t)(rel / bpg) * 2 * segcap
-                         : (size_t)(rel / bpg) * 2 * segcap;
-    TeamVars *tv = tvs + group;
+    seg_off += (size_t)(rel / bpg) * 2 * (size_t)segcap;
     __shared__ RoundSlot view;
+    TeamVars *tv = tvs + group;
     TeamMem m;
     m.slot = tv->slot;
     m.view = &view;
     m.stats = &tv->n_pop;
     m.seg = segs + seg_off;
     m.segcap = segcap;
-    uint32_t *head = qhead + cls;
-    const int nq = shape.nq[cls], qoff = shape.qoff[cls];
     GroupTeam team(m, tv, bpg, rel % bpg);
-    group_loop(qs, rs, team, tv, head, nq, qoff, pdb3, pdb4);
+    group_loop(qs, rs, team, tv, qhead + cls, shape.nq[cls], shape.qoff[cls], shape.budget_ticks, pdb3, pdb4);
 }
 
 }  // namespace ua

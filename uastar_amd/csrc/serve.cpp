@@ -3,14 +3,16 @@
 // Submitted queries wait in a pending queue. Each ua_serve_step():
 //   1. admits pending queries while a team slot is free and their memory estimate
 //      (best-cost table or hash, heap storage, resume record) fits the device pool;
-//   2. runs ONE bounded cooperative launch of k_teams over every resident query:
-//      each query advances up to `round_budget` GA* rounds, then saves its heap tops
-//      and counters to its resume record (state stays resident in HBM);
+//   2. runs ONE bounded cooperative launch of k_teams over every resident query, in
+//      admission order: each group runs queries until it has been busy for the launch's
+//      time budget, then suspends the query it runs (heap tops and counters go to the
+//      query's resume record; the state stays resident in HBM). A query the launch does
+//      not reach waits for the next one;
 //   3. retires the queries that finished (found / no path / error), frees their memory.
 // Queries that arrive while others are on the device are admitted at the next step,
-// so their work overlaps on the device with the queries already in flight. The round
-// budget adapts so that one launch lasts a few milliseconds. A query that outgrows
-// its heap or hash is restarted with a larger plan.
+// so their work overlaps on the device with the queries already in flight. A heavy
+// query runs for the whole launch on its team; light ones finish and make room. A query
+// that outgrows its heap or hash is restarted with a larger plan.
 #include <algorithm>
 #include <chrono>
 #include <deque>
@@ -92,12 +94,18 @@ struct ServeEngine {
     uint64_t next_ticket = 1;
     Pool pool;
     bool pool_ready = false;
-    int round_budget[2] = {32, 128};  // per class: 0 = puzzles, 1 = grids
+    int64_t budget_ticks = 400000;  // device time per launch: 4 ms of s_memrealtime (100 MHz)
     int64_t launches = 0;
     double kernel_ms = 0;
 };
 
-static int cls_of(const ua_query &q) { return q.kind == UA_QUERY_PUZZLE ? 0 : 1; }
+// the team-width class of a planned job, and of a query before planning
+static int64_t cells_of(const ua_query &q) {
+    return q.kind == UA_QUERY_GRID && q.grid ? (int64_t)q.grid->H * q.grid->W : 0;
+}
+static int cls_of_query(ua_ctx *c, const ua_query &q) {
+    return class_of(c, q.kind == UA_QUERY_PUZZLE ? KIND_PUZZLE : KIND_GRID, cells_of(q));
+}
 
 void ua_serve_release(ua_ctx *c) {
     if (!c || !c->serve) return;
@@ -117,7 +125,9 @@ ServeEngine *engine(ua_ctx *c) {
 // Plans job j for its current scale and sets its device pointers inside the pool.
 int plan_job(ua_ctx *c, ServeEngine *e, Job &j, bool *fits) {
     *fits = false;
-    const int heaps = 4 * team_bpg(c, j.q.kind == UA_QUERY_PUZZLE ? KIND_PUZZLE : KIND_GRID);
+    const int heaps = j.q.kind == UA_QUERY_PUZZLE
+                          ? team_waves(c, KIND_PUZZLE, 0)
+                          : team_waves(c, KIND_GRID, j.q.grid ? (int64_t)j.q.grid->H * j.q.grid->W : 0);
     int rc;
     if (j.q.kind == UA_QUERY_GRID) {
         rc = grid_fill_desc(c, j.q.grid, j.q.sx, j.q.sy, j.q.ex, j.q.ey, heaps, j.scale, j.d, j.plan);
@@ -201,14 +211,18 @@ extern "C" int ua_serve_step(ua_ctx *c, int64_t *n_active, int64_t *n_pending) {
         HIPCHK(c, e->pool.init(budget));
         e->pool_ready = true;
     }
+    // resident queries per team width: about twice the teams of that width one launch
+    // could hold if it had the device to itself
     int per_cu = 0;
     HIPCHK(c, teams_max_blocks(&per_cu));
     const int resident = c->num_cu * std::max(1, per_cu);
-    // resident queries per class: about two per group the launch can hold
-    const size_t cap_cls[2] = {(size_t)std::max(1, 2 * resident / 2 / team_bpg(c, KIND_PUZZLE)),
-                               (size_t)std::max(1, 2 * resident / team_bpg(c, KIND_GRID))};
-    size_t n_cls[2] = {0, 0};
-    for (const Job &j : e->active) n_cls[cls_of(j.q)]++;
+    size_t cap_cls[kClasses], n_cls[kClasses] = {0, 0, 0};
+    for (int k = 0; k < kClasses; ++k) {
+        const int kind = k == CLS_PUZZLE ? KIND_PUZZLE : KIND_GRID;
+        const int64_t cells = k == CLS_GRID_LARGE ? c->tune.grid_large_cells : 0;
+        cap_cls[k] = (size_t)std::max(1, 2 * resident / (team_waves(c, kind, cells) / kBlockWaves));
+    }
+    for (const Job &j : e->active) n_cls[cls_of_query(c, j.q)]++;
 
     // 1. admission (in arrival order; a class at its cap waits, the other can still be admitted)
     {
@@ -221,7 +235,7 @@ extern "C" int ua_serve_step(ua_ctx *c, int64_t *n_active, int64_t *n_pending) {
                 j = e->pending.front();
                 e->pending.pop_front();
             }
-            int k = cls_of(j.q);
+            int k = cls_of_query(c, j.q);
             if (n_cls[k] >= cap_cls[k]) {
                 later.push_back(j);
                 continue;
@@ -247,13 +261,17 @@ extern "C" int ua_serve_step(ua_ctx *c, int64_t *n_active, int64_t *n_pending) {
     if (!e->active.empty()) {
         // 2. one bounded launch over every resident query
         std::vector<QueryDesc> descs;
+        std::vector<QueryResult> init(e->active.size());
         descs.reserve(e->active.size());
-        for (Job &j : e->active) {
-            j.d.round_budget = e->round_budget[cls_of(j.q)];
+        for (size_t i = 0; i < e->active.size(); ++i) {
+            Job &j = e->active[i];
+            j.d.round_budget = 0;
             descs.push_back(j.d);
+            init[i].status = ST_SUSPENDED;  // what a query the launch does not reach reports
+            init[i].rounds = j.rounds;
         }
         std::vector<QueryResult> res;
-        int rc = run_batch(c, descs, res, 2);
+        int rc = run_batch(c, descs, res, 2, e->budget_ticks, &init);
         if (rc) return rc;
         double launch_ms = c->last_kernel_ms;
         e->launches++;
@@ -261,16 +279,13 @@ extern "C" int ua_serve_step(ua_ctx *c, int64_t *n_active, int64_t *n_pending) {
         // 3. retirement
         std::vector<Job> still;
         auto now = Clock::now();
-        double cls_ms[2] = {0, 0}, cls_rounds[2] = {0, 0};
         for (size_t i = 0; i < e->active.size(); ++i) {
             Job &j = e->active[i];
             const QueryResult &r = res[i];
-            const double dms = (double)(r.t_end - r.t_start) / 1e5;
+            const double dms = r.t_end > r.t_start ? (double)(r.t_end - r.t_start) / 1e5 : 0.0;
             j.launches++;
             j.kernel_ms += launch_ms;
             j.device_ms += dms;
-            cls_ms[cls_of(j.q)] += dms;
-            cls_rounds[cls_of(j.q)] += (double)(r.rounds - j.rounds);
             j.rounds = r.rounds;
             if (r.status == ST_SUSPENDED) {
                 still.push_back(j);
@@ -292,13 +307,6 @@ extern "C" int ua_serve_step(ua_ctx *c, int64_t *n_active, int64_t *n_pending) {
             complete(e, j, out);
         }
         e->active.swap(still);
-        // keep each class's share of a launch near 4 ms: budget = 4 ms / (ms per round)
-        for (int k = 0; k < 2; ++k) {
-            if (cls_rounds[k] <= 0) continue;
-            double ms_per_round = cls_ms[k] / cls_rounds[k];
-            int want = (int)std::min(65536.0, std::max(8.0, 4.0 / std::max(ms_per_round, 1e-4)));
-            e->round_budget[k] = (e->round_budget[k] + want) / 2;
-        }
     }
     if (n_active) *n_active = (int64_t)e->active.size();
     if (n_pending) {
@@ -332,11 +340,11 @@ extern "C" int ua_serve_drain(ua_ctx *c) {
     }
 }
 
-extern "C" int ua_serve_stats(ua_ctx *c, int64_t *launches, double *kernel_ms, int32_t *round_budget) {
+extern "C" int ua_serve_stats(ua_ctx *c, int64_t *launches, double *kernel_ms, int32_t *budget_us) {
     if (!c) return UA_ERR_INVALID;
     ServeEngine *e = engine(c);
     if (launches) *launches = e->launches;
     if (kernel_ms) *kernel_ms = e->kernel_ms;
-    if (round_budget) *round_budget = e->round_budget[1];
+    if (budget_us) *budget_us = (int32_t)(e->budget_ticks / 100);
     return UA_OK;
 }
