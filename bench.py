@@ -343,6 +343,7 @@ def run_serve_mixed(args):
         "exact_mismatches_serve_vs_plain": mismatch,
         "errors": errors,
         "serve_launches": launches,
+        "serve_widened": st1["widened"] - st0["widened"],
         "tuning": tuning,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),

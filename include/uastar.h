@@ -127,9 +127,10 @@ int ua_serve_step(ua_ctx *ctx, int64_t *n_active, int64_t *n_pending);
 int ua_serve_poll(ua_ctx *ctx, uint64_t *tickets, ua_result *results, int64_t cap, int64_t *n_out);
 /* steps until nothing is pending or active */
 int ua_serve_drain(ua_ctx *ctx);
-/* launches so far, their summed duration (HIP events) and the device-time budget of one
- * launch in microseconds */
-int ua_serve_stats(ua_ctx *ctx, int64_t *launches, double *kernel_ms, int32_t *budget_us);
+/* launches so far, their summed duration (HIP events), the device-time budget of one
+ * launch in microseconds and the queries widened so far (long-running 15-puzzles move
+ * to a team of a quarter of the device) */
+int ua_serve_stats(ua_ctx *ctx, int64_t *launches, double *kernel_ms, int32_t *budget_us, int64_t *widened);
 
 /* ---- autotuning (no reference counterpart: uAstar fixes its launch shape and table
  * sizes at compile time, pathway/GPU-solver.hpp:5-17 and puzzle/GPU-solver.cuh:12-22).

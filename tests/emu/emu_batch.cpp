@@ -12,7 +12,7 @@ extern "C" int emu_batch(const ua::QueryDesc *qs, ua::QueryResult *rs, int nq, i
     const int segcap = waves * 64 * 8;
     std::vector<uint64_t> segs((size_t)2 * segcap, 0);
     ua::TeamVars tv{};
-    uint32_t qhead[ua::kClasses] = {0, 0, 0};
+    uint32_t qhead[ua::kClasses] = {};
     ua::LaunchShape shape{};
     shape.groups[ua::CLS_GRID] = 1;  // one one-block group takes every query
     for (int k = 0; k < ua::kClasses; ++k) shape.bpg[k] = 1;

@@ -24,7 +24,7 @@ class QueryDesc(C.Structure):
     _fields_ = [(n, C.c_int32) for n in ("kind", "n", "H", "W", "sx", "sy", "ex", "ey", "wpr", "ybits", "cbits",
                                           "qshift", "heap_cap", "hash_log2", "max_rounds", "heaps")] + [
         ("bitmap", C.c_void_p), ("best", C.c_void_p), ("vals", C.c_void_p), ("heap", C.c_void_p),
-        ("start_state", C.c_uint64), ("resume", C.c_void_p), ("round_budget", C.c_int32), ("pad2", C.c_int32)]
+        ("start_state", C.c_uint64), ("resume", C.c_void_p), ("round_budget", C.c_int32), ("cls", C.c_int32)]
 
 RESUME_WORDS_4 = 8 + 4 * 130
 

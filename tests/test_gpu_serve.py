@@ -107,3 +107,21 @@ def test_serve_same_alone_and_in_mix(sctx, oracle):
     for g2 in others:
         g2.free()
     grid.free()
+
+
+def test_serve_widens_long_puzzle(sctx):
+    """A heavy 15-puzzle moves to a wide team partway through and still returns the
+    CPU A* move count; light queries served beside it are unaffected."""
+    ctx = sctx
+    rows = sorted(json.load(open(os.path.join(GOLDEN, "c3_puzzles.json"))), key=lambda r: -r["expanded"])
+    heavy = rows[0]
+    light = [r for r in rows if r["expanded"] < 100_000][:4]
+    w0 = ctx.serve_stats()["widened"]
+    tickets = {ctx.serve_submit(ctx.puzzle_query(4, heavy["board"])): heavy["steps"]}
+    for r in light:
+        tickets[ctx.serve_submit(ctx.puzzle_query(4, r["board"]))] = r["steps"]
+    _say(f"heavy puzzle seed {heavy['seed']} ({heavy['expanded']} CPU expansions) + {len(light)} light")
+    got = _drain(ctx, len(tickets))
+    for t, steps in tickets.items():
+        assert got[t]["found"] and got[t]["steps"] == steps, (t, got[t], steps)
+    assert ctx.serve_stats()["widened"] > w0

@@ -101,7 +101,8 @@ def lib():
             "ua_serve_step": (C.c_int, [vp, C.POINTER(i64), C.POINTER(i64)]),
             "ua_serve_poll": (C.c_int, [vp, C.POINTER(u64), C.POINTER(Result), i64, C.POINTER(i64)]),
             "ua_serve_drain": (C.c_int, [vp]),
-            "ua_serve_stats": (C.c_int, [vp, C.POINTER(i64), C.POINTER(C.c_double), C.POINTER(i32)]),
+            "ua_serve_stats": (C.c_int, [vp, C.POINTER(i64), C.POINTER(C.c_double), C.POINTER(i32),
+                                         C.POINTER(i64)]),
             "ua_debug_heap_selftest": (C.c_int, [vp, vp, C.c_int, vp, i64, C.c_int, vp, i64,
                                                  C.POINTER(i32)]),
             "ua_autotune": (C.c_int, [vp, C.POINTER(Query), i64, C.POINTER(Tuning)]),
@@ -317,9 +318,9 @@ class Context:
         self._check(lib().ua_serve_drain(self.h))
 
     def serve_stats(self):
-        n, ms, rb = C.c_int64(), C.c_double(), C.c_int32()
-        self._check(lib().ua_serve_stats(self.h, C.byref(n), C.byref(ms), C.byref(rb)))
-        return {"launches": n.value, "kernel_ms": ms.value, "budget_us": rb.value}
+        n, ms, rb, wd = C.c_int64(), C.c_double(), C.c_int32(), C.c_int64()
+        self._check(lib().ua_serve_stats(self.h, C.byref(n), C.byref(ms), C.byref(rb), C.byref(wd)))
+        return {"launches": n.value, "kernel_ms": ms.value, "budget_us": rb.value, "widened": wd.value}
 
     # ---- autotuning
     def autotune(self, queries) -> dict:

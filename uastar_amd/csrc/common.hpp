@@ -105,16 +105,17 @@ struct QueryDesc {
     uint64_t start_state;    // puzzle
     uint64_t *resume;        // serving: resume record (resume_words(heaps) u64), or null
     int32_t round_budget;    // serving: at most this many rounds per launch (0 = no limit)
-    int32_t pad2;
+    int32_t cls;             // host side: instance class of the launch (CLS_*)
 };
 
 // The search kernel runs blocks of 16 wavefronts (1024 threads). A team is a group of
 // whole blocks: 16, 32, 64 ... waves (= heaps).
 constexpr int kBlockWaves = 16;
 
-// Instance classes of a batched / served launch, each with its own team width.
-constexpr int kClasses = 3;
-enum : int { CLS_PUZZLE = 0, CLS_GRID = 1, CLS_GRID_LARGE = 2 };
+// Instance classes of a batched / served launch, each with its own team width. CLS_WIDE
+// holds served queries that ran long and were widened to a large team.
+constexpr int kClasses = 4;
+enum : int { CLS_PUZZLE = 0, CLS_GRID = 1, CLS_GRID_LARGE = 2, CLS_WIDE = 3 };
 
 // Shape of one k_teams launch. Class c has groups[c] groups of bpg[c] blocks (classes
 // in order of c in block order); its groups pull the queries qs[qoff[c] .. qoff[c] +

@@ -62,6 +62,7 @@ int grid_fill_desc(ua_ctx *c, const ua_grid *g, int sx, int sy, int ex, int ey, 
     q.qshift = std::max(0, q.cbits - 10);
     q.bitmap = g->bm;
     q.heaps = heaps;
+    q.cls = class_of(c, KIND_GRID, (int64_t)g->H * g->W);
     q.max_rounds = c->cfg.max_rounds > 0 ? c->cfg.max_rounds : (1 << 26);
     int64_t cells = (int64_t)g->H * g->W;
     int64_t items = std::max<int64_t>(65536, cells / 8) * heap_scale;  // open-list capacity
@@ -117,6 +118,7 @@ int puzzle_fill_desc(ua_ctx *c, int n, const uint8_t *tiles, int heaps, int hash
     q.start_state = pack_board(n, tiles);
     q.hash_log2 = hash_log2;
     q.heaps = heaps;
+    q.cls = CLS_PUZZLE;
     q.max_rounds = c->cfg.max_rounds > 0 ? c->cfg.max_rounds : (1 << 26);
     int64_t items = std::max<int64_t>(65536, (1ll << hash_log2) / 2);
     int64_t per_heap = (items + 64ll * heaps - 1) / (64ll * heaps);
@@ -507,7 +509,7 @@ int run_batch(ua_ctx *c, std::vector<QueryDesc> &descs, std::vector<QueryResult>
         shape.qoff[k] = (int)order.size();
         for (int i = 0; i < nq; ++i) {
             const QueryDesc &q = descs[i];
-            if (class_of(c, q.kind, q.kind == KIND_GRID ? (int64_t)q.H * q.W : 0) != k) continue;
+            if (q.cls != k) continue;
             int bpg = q.heaps / kBlockWaves;
             if (bpg < 1 || q.heaps % kBlockWaves || (shape.bpg[k] && shape.bpg[k] != bpg) || bpg > resident)
                 return fail(c, UA_ERR_INVALID, "query %d planned for %d heaps, not a team width of its class", i,
@@ -519,12 +521,19 @@ int run_batch(ua_ctx *c, std::vector<QueryDesc> &descs, std::vector<QueryResult>
         if (!shape.bpg[k]) shape.bpg[k] = 1;
         shape.segcap[k] = shape.bpg[k] * kBlockWaves * 64 * 8;  // at most 8 successors per pop
     }
-    double demand = 0;
-    for (int k = 0; k < kClasses; ++k) demand += (double)shape.nq[k] * shape.bpg[k];
+    // widened queries first (a group each), then the rest in proportion to demand
     int used = 0;
+    if (shape.nq[CLS_WIDE]) {
+        shape.groups[CLS_WIDE] = std::max(1, std::min(shape.nq[CLS_WIDE], resident / 2 / shape.bpg[CLS_WIDE]));
+        used = shape.groups[CLS_WIDE] * shape.bpg[CLS_WIDE];
+    }
+    double demand = 0;
+    for (int k = 0; k < kClasses; ++k)
+        if (k != CLS_WIDE) demand += (double)shape.nq[k] * shape.bpg[k];
+    const int rest = resident - used;
     for (int k = 0; k < kClasses; ++k) {
-        if (!shape.nq[k]) continue;
-        int want = (int)(resident * shape.nq[k] * shape.bpg[k] / demand);
+        if (!shape.nq[k] || k == CLS_WIDE) continue;
+        int want = (int)(rest * shape.nq[k] * shape.bpg[k] / demand);
         shape.groups[k] = std::max(1, std::min(shape.nq[k], want / shape.bpg[k]));
         used += shape.groups[k] * shape.bpg[k];
     }

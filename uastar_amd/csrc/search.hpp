@@ -191,8 +191,9 @@ struct GridProb {
         return goal != SENT && goal <= ((lb >> q.cbits) << q.qshift);
     }
     // Per-lane expansion of one popped item; successors land in s[0..7] (SENT = none).
-    // Memory is touched in two waves of independent accesses: the cell's best word and
-    // the 3x3 bitmap window around it (six words), then one atomicMin per legal move.
+    // Memory is touched in three waves of independent accesses: the cell's best word and
+    // the 3x3 bitmap window around it (six words); the legal neighbours' best words; an
+    // atomicMin for each neighbour the move would improve.
     __device__ __forceinline__ void expand(const Scope &sc, uint64_t it, uint64_t (&s)[MAXS], int &cnt, uint64_t &goalmin,
                            uint32_t &err, int64_t &n_exp, int64_t &n_gen, int64_t &n_ins) const {
         uint64_t cell = it & cmask;
@@ -230,7 +231,10 @@ struct GridProb {
             if (y + 1 < q.W && ((whi[i] >> ((y + 1) & 63)) & 1ull)) win |= 1u << (3 * i + 2);
         }
         if (!((win >> 4) & 1u)) return;  // source-cell mask rule
-        uint64_t old[8];
+        // read the neighbours' best words, then relax with an atomic only where the new
+        // cost is lower than what was read (best words only ever decrease, so a skipped
+        // atomic could not have succeeded); most successors on a grid are duplicates
+        uint64_t w[8];
         uint32_t live = 0;
 #pragma unroll
         for (int d = 0; d < 8; ++d) {
@@ -245,14 +249,16 @@ struct GridProb {
                 continue;
             }
             live |= 1u << d;
-            old[d] = sc.fmin(&q.best[(uint64_t)nx * q.W + ny], (ng << 4) | (uint64_t)d);
+            w[d] = sc.load(&q.best[(uint64_t)nx * q.W + ny]);
         }
 #pragma unroll
         for (int d = 0; d < 8; ++d) {
             uint64_t ng = gk + (d < 4 ? KEY_ONE : KEY_SQRT2);
-            if (!((live >> d) & 1u) || (old[d] >> 4) <= ng) continue;
-            ++n_ins;
+            if (!((live >> d) & 1u) || (w[d] >> 4) <= ng) continue;
             int nx = x + kDX[d], ny = y + kDY[d];
+            uint64_t old = sc.fmin(&q.best[(uint64_t)nx * q.W + ny], (ng << 4) | (uint64_t)d);
+            if ((old >> 4) <= ng) continue;
+            ++n_ins;
             if ((uint64_t)nx * q.W + ny == goal_id) {
                 goalmin = goalmin < ng ? goalmin : ng;
             } else {

@@ -77,7 +77,9 @@ struct Job {
     Clock::time_point submit;
     QueryDesc d{};
     Plan plan;
-    size_t off = 0, bytes = 0;
+    size_t off = 0, bytes = 0;    // pool region: best table / hash, then heaps and resume record
+    size_t off2 = 0, bytes2 = 0;  // widened: heaps and resume record of the wide team
+    bool wide = false;
     int scale = 1;      // grid heap scale / puzzle hash growth steps
     int launches = 0;
     int64_t rounds = 0;  // rounds completed so far
@@ -95,6 +97,9 @@ struct ServeEngine {
     Pool pool;
     bool pool_ready = false;
     int64_t budget_ticks = 400000;  // device time per launch: 4 ms of s_memrealtime (100 MHz)
+    double widen_after_ms = 25.0;   // a puzzle that has run this long moves to a wide team
+    int max_wide = 2;               // widened queries at a time (each takes a quarter of the device)
+    int64_t widened = 0;
     int64_t launches = 0;
     double kernel_ms = 0;
 };
@@ -165,6 +170,41 @@ int plan_job(ua_ctx *c, ServeEngine *e, Job &j, bool *fits) {
         HIPCHK(c, hipMemsetAsync(j.d.vals, 0xFF, j.plan.vals_bytes, c->st));
     }
     HIPCHK(c, hipMemsetAsync(j.d.resume, 0, 64, c->st));
+    return UA_OK;
+}
+
+// Moves a long-running served puzzle to a wide team of `waves` heaps. Its heaps keep
+// their contents as the first heaps of the wider team, the new heaps start empty and
+// fill from the rounds' insert distribution; its hash table stays where it is. Runs
+// between launches. Without pool memory for the wider heaps the query stays as it is.
+int widen(ua_ctx *c, ServeEngine *e, Job &j, int waves) {
+    const int w0 = j.d.heaps;
+    const size_t per_heap = (size_t)(j.d.heap_cap - 1) * 64 * 8;
+    const size_t heap_bytes = (size_t)waves * per_heap;
+    const size_t bytes2 = align256(heap_bytes) + align256((size_t)resume_words(waves) * 8);
+    size_t off2;
+    if (!e->pool.alloc(bytes2, &off2)) return UA_OK;
+    char *base2 = e->pool.arena.as<char>() + off2;
+    uint64_t *heap2 = reinterpret_cast<uint64_t *>(base2);
+    uint64_t *rec2 = reinterpret_cast<uint64_t *>(base2 + align256(heap_bytes));
+    HIPCHK(c, hipMemcpyAsync(heap2, j.d.heap, (size_t)w0 * per_heap, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(rec2, j.d.resume, (size_t)resume_words(w0) * 8, hipMemcpyDeviceToDevice, c->st));
+    // the new heaps: empty root and partial batches (all ones), n = pc = 0
+    std::vector<uint64_t> fresh((size_t)(waves - w0) * RESUME_PER_HEAP, ~0ull);
+    for (int h = 0; h < waves - w0; ++h) fresh[(size_t)h * RESUME_PER_HEAP + 128] = fresh[(size_t)h * RESUME_PER_HEAP + 129] = 0;
+    HIPCHK(c, hipMemcpyAsync(rec2 + resume_words(w0), fresh.data(), fresh.size() * 8, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    const size_t keep = align256(j.plan.best_bytes) + align256(j.plan.vals_bytes);
+    e->pool.release(j.off + keep, j.bytes - keep);  // the narrow team's heaps and record
+    j.bytes = keep;
+    j.off2 = off2;
+    j.bytes2 = bytes2;
+    j.plan.heap_bytes = heap_bytes;
+    j.d.heaps = waves;
+    j.d.heap = heap2;
+    j.d.resume = rec2;
+    j.d.cls = CLS_WIDE;
+    j.wide = true;
     return UA_OK;
 }
 
@@ -292,6 +332,9 @@ extern "C" int ua_serve_step(ua_ctx *c, int64_t *n_active, int64_t *n_pending) {
                 continue;
             }
             e->pool.release(j.off, j.bytes);
+            if (j.bytes2) e->pool.release(j.off2, j.bytes2);
+            j.bytes2 = 0;
+            j.wide = false;
             if (r.status == ST_CAPACITY && j.scale < 256) {  // restart with a larger plan
                 j.scale *= 4;
                 j.rounds = 0;
@@ -307,6 +350,19 @@ extern "C" int ua_serve_step(ua_ctx *c, int64_t *n_active, int64_t *n_pending) {
             complete(e, j, out);
         }
         e->active.swap(still);
+        // 4. widen long-running puzzles (frontier-bound: more heaps pop more per round)
+        const int wide_waves = kBlockWaves * std::max(1, resident / 4);
+        int n_wide = 0;
+        for (const Job &j : e->active) n_wide += j.wide;
+        for (Job &j : e->active) {
+            if (n_wide >= e->max_wide) break;
+            if (j.wide || j.q.kind != UA_QUERY_PUZZLE || j.device_ms < e->widen_after_ms || j.d.heaps >= wide_waves)
+                continue;
+            int wrc = widen(c, e, j, wide_waves);
+            if (wrc) return wrc;
+            n_wide += j.wide;
+            e->widened += j.wide;
+        }
     }
     if (n_active) *n_active = (int64_t)e->active.size();
     if (n_pending) {
@@ -340,11 +396,13 @@ extern "C" int ua_serve_drain(ua_ctx *c) {
     }
 }
 
-extern "C" int ua_serve_stats(ua_ctx *c, int64_t *launches, double *kernel_ms, int32_t *budget_us) {
+extern "C" int ua_serve_stats(ua_ctx *c, int64_t *launches, double *kernel_ms, int32_t *budget_us,
+                              int64_t *widened) {
     if (!c) return UA_ERR_INVALID;
     ServeEngine *e = engine(c);
     if (launches) *launches = e->launches;
     if (kernel_ms) *kernel_ms = e->kernel_ms;
     if (budget_us) *budget_us = (int32_t)(e->budget_ticks / 100);
+    if (widened) *widened = e->widened;
     return UA_OK;
 }
