@@ -98,6 +98,9 @@ int ua_grid_path(ua_ctx *ctx, int32_t *ids, int64_t cap, int64_t *len);
 int ua_puzzle_build_pdb(ua_ctx *ctx, int n);
 /* Copies the device PDB (all partitions concatenated) to the host. */
 int ua_puzzle_download_pdb(ua_ctx *ctx, int n, uint8_t *host, uint64_t size);
+/* Uploads a PDB (all partitions concatenated, e.g. read from uAstar's database_*.bin
+ * cache files, puzzle/database.cpp:86-122) instead of building it; call before solving. */
+int ua_puzzle_load_pdb(ua_ctx *ctx, int n, const uint8_t *host, uint64_t size);
 /* Plain solve of one board (row-major, 0 = blank): replaces GPUPuzzleSolver<N>::solve
  * (puzzle/GPU-solver.cuh:197-274). Unsolvable boards return UA_NO_PATH at once. */
 int ua_puzzle_solve(ua_ctx *ctx, int n, const uint8_t *tiles, ua_result *res);

@@ -10,6 +10,7 @@ import os
 import numpy as np
 import pytest
 
+import uastar_amd as U
 from tests.helpers import goal_board, replay_moves
 
 pytestmark = pytest.mark.gpu
@@ -24,6 +25,19 @@ def test_pdb3_byte_exact(ctx, oracle):
     dev = ctx.puzzle_download_pdb(3)
     ref = oracle.pdb_build(3, list(range(1, 9)), 1)
     assert np.array_equal(dev, ref)
+
+
+def test_pdb_load_from_host(ctx, oracle):
+    """A PDB uploaded from host memory (as from uAstar's cache files) serves the solver."""
+    ref = oracle.pdb_build(3, list(range(1, 9)), 1)
+    with U.Context(0) as c2:
+        with pytest.raises(U.UAError):
+            c2.puzzle_load_pdb(3, ref[:-1])  # wrong size
+        c2.puzzle_load_pdb(3, ref)
+        assert np.array_equal(c2.puzzle_download_pdb(3), ref)
+        board = oracle.puzzle_random(3, 777)
+        steps, _, _ = oracle.puzzle_solve(3, board, ref, "astar")
+        assert c2.puzzle_solve(3, board)["steps"] == steps
 
 
 def test_pdb4_matches_digest(ctx):

@@ -93,6 +93,7 @@ def lib():
             "ua_grid_path": (C.c_int, [vp, vp, i64, C.POINTER(i64)]),
             "ua_puzzle_build_pdb": (C.c_int, [vp, C.c_int]),
             "ua_puzzle_download_pdb": (C.c_int, [vp, C.c_int, vp, u64]),
+            "ua_puzzle_load_pdb": (C.c_int, [vp, C.c_int, vp, u64]),
             "ua_puzzle_solve": (C.c_int, [vp, C.c_int, vp, C.POINTER(Result)]),
             "ua_puzzle_moves": (C.c_int, [vp, vp, i64, C.POINTER(i64)]),
             "ua_batch_solve": (C.c_int, [vp, C.POINTER(Query), i64, C.POINTER(Result)]),
@@ -254,6 +255,10 @@ class Context:
         out = np.empty(size, np.uint8)
         self._check(lib().ua_puzzle_download_pdb(self.h, n, out.ctypes.data, size))
         return out
+
+    def puzzle_load_pdb(self, n: int, pdb: np.ndarray):
+        pdb = np.ascontiguousarray(pdb, dtype=np.uint8)
+        self._check(lib().ua_puzzle_load_pdb(self.h, n, pdb.ctypes.data, pdb.size))
 
     def puzzle_solve(self, n: int, tiles) -> dict:
         t = np.ascontiguousarray(tiles, dtype=np.uint8).reshape(-1)

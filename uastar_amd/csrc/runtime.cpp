@@ -402,6 +402,17 @@ extern "C" int ua_puzzle_build_pdb(ua_ctx *c, int n) {
     return ensure_pdb(c, n);
 }
 
+extern "C" int ua_puzzle_load_pdb(ua_ctx *c, int n, const uint8_t *host, uint64_t size) {
+    if (!c || !host) return fail(c, UA_ERR_INVALID, "null argument");
+    if (n != 3 && n != 4) return fail(c, UA_ERR_INVALID, "puzzle side must be 3 or 4");
+    if (size != pdb_bytes(n)) return fail(c, UA_ERR_INVALID, "a %dx%d PDB holds %llu bytes", n, n,
+                                          (unsigned long long)pdb_bytes(n));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    HIPCHK(c, c->pdb[n].ensure(pdb_bytes(n)));
+    HIPCHK(c, hipMemcpy(c->pdb[n].p, host, pdb_bytes(n), hipMemcpyHostToDevice));
+    return UA_OK;
+}
+
 extern "C" int ua_puzzle_download_pdb(ua_ctx *c, int n, uint8_t *host, uint64_t size) {
     if (!c || !host) return fail(c, UA_ERR_INVALID, "null argument");
     int rc = ensure_pdb(c, n);
