@@ -215,12 +215,15 @@ def serve_pass(ctx, specs, arrivals_ms):
     t0 = time.perf_counter()
     i = 0
     last_say = t0
+    step_wall = 0.0  # host time inside serve_step: launch + host work around it
     while len(done) < n:
         now = (time.perf_counter() - t0) * 1e3
         while i < n and arrivals_ms[i] <= now:
             ticket_of[ctx.serve_submit(queries[i])] = i
             i += 1
+        ts = time.perf_counter()
         active, pending = ctx.serve_step()
+        step_wall += time.perf_counter() - ts
         tnow = (time.perf_counter() - t0) * 1e3
         for t, r in ctx.serve_poll():
             done[ticket_of[t]] = (r, tnow)
@@ -232,7 +235,7 @@ def serve_pass(ctx, specs, arrivals_ms):
     res = [done[k][0] for k in range(n)]
     lat = [done[k][1] - arrivals_ms[k] for k in range(n)]
     span = max(done[k][1] for k in range(n)) - arrivals_ms[0]
-    return res, lat, span
+    return res, lat, span, step_wall * 1e3
 
 
 def cpu_baseline_mixed(specs, budget_s=25.0, threads=None):
@@ -306,7 +309,7 @@ def run_serve_mixed(args):
     arrivals = [a - arrivals[0] for a in arrivals]
     st0 = ctx.serve_stats()
     say(f"serve pass: Poisson arrivals at {rate:.1f} queries/s ({args.load:g} x plain capacity)")
-    sres, slat, span = serve_pass(ctx, specs, arrivals)
+    sres, slat, span, step_wall_ms = serve_pass(ctx, specs, arrivals)
     st1 = ctx.serve_stats()
     serve_qps = len(specs) / (span / 1e3)
     plat = fifo_latency(arrivals, svc)
@@ -344,6 +347,10 @@ def run_serve_mixed(args):
         "errors": errors,
         "serve_launches": launches,
         "serve_widened": st1["widened"] - st0["widened"],
+        # host time per serving launch outside the kernel (admission, descriptor and
+        # result copies, launch, retirement), and per plain query outside its kernel
+        "host_overhead_ms": {"per_serve_launch": (step_wall_ms - kms) / max(1, launches),
+                             "per_plain_query": (sum(svc) - sum(r["kernel_ms"] for r in pres)) / len(pres)},
         "tuning": tuning,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
