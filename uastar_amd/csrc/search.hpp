@@ -190,79 +190,90 @@ struct GridProb {
     __device__ bool goal_done(uint64_t goal, uint64_t lb) const {
         return goal != SENT && goal <= ((lb >> q.cbits) << q.qshift);
     }
+    // Octile key from the absolute offsets to the goal (each below 2^31).
+    __device__ __forceinline__ static uint64_t oct(uint32_t adx, uint32_t ady) {
+        uint32_t mn = adx < ady ? adx : ady, mx = adx < ady ? ady : adx;
+        return ((uint64_t)(mx - mn) << 36) + (uint64_t)mn * KEY_SQRT2;
+    }
+    __device__ __forceinline__ static uint32_t uabs(int v) { return (uint32_t)(v < 0 ? -v : v); }
+
     // Per-lane expansion of one popped item; successors land in s[0..7] (SENT = none).
     // Memory is touched in three waves of independent accesses: the cell's best word and
     // the 3x3 bitmap window around it (six words); the legal neighbours' best words; an
-    // atomicMin for each neighbour the move would improve.
-    __device__ __forceinline__ void expand(const Scope &sc, uint64_t it, uint64_t (&s)[MAXS], int &cnt, uint64_t &goalmin,
-                           uint32_t &err, int64_t &n_exp, int64_t &n_gen, int64_t &n_ins) const {
-        uint64_t cell = it & cmask;
-        int x = (int)(cell >> q.ybits), y = (int)(cell & ymask);
+    // atomicMin for each neighbour the move would improve. Coordinates and offsets are
+    // 32-bit; a neighbour's best word is addressed from the cell's (row offsets +-W).
+    template <class SC>
+    __device__ __forceinline__ void expand(const SC &sc, uint64_t it, uint64_t (&s)[MAXS], int &cnt,
+                                           uint64_t &goalmin, uint32_t &err, int64_t &n_exp, int64_t &n_gen,
+                                           int64_t &n_ins) const {
+        const uint64_t cell = it & cmask;
+        const int x = (int)(cell >> q.ybits), y = (int)(cell & ymask);
         if (x >= q.H || y >= q.W) {  // never produced by a correct round: fail the query, touch nothing
             err |= FLAG_ABORT;
             return;
         }
-        uint64_t id = (uint64_t)x * q.W + y;
+        uint64_t *const bp = q.best + ((uint64_t)(uint32_t)x * (uint32_t)q.W + (uint32_t)y);
         // issue every load first
-        uint64_t bw = sc.load(&q.best[id]);
+        const uint64_t bw = sc.load(bp);
         const int clo = y > 0 ? y - 1 : 0, chi = y + 1 < q.W ? y + 1 : q.W - 1;
         uint64_t wlo[3], whi[3];
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
             int r = x + i - 1;
             r = r < 0 ? 0 : (r >= q.H ? q.H - 1 : r);
-            const uint64_t *row = q.bitmap + (size_t)r * q.wpr;
+            const uint64_t *row = q.bitmap + (uint64_t)(uint32_t)r * (uint32_t)q.wpr;
             wlo[i] = row[clo >> 6];
             whi[i] = row[chi >> 6];
         }
-        uint64_t gk = bw >> 4;
-        uint64_t fk = gk + octile_key(x, y, q.ex, q.ey);
-        if ((fk >> q.qshift) != (it >> q.cbits)) return;  // stale: a better entry exists
+        const int gx = x - q.ex, gy = y - q.ey;  // offsets to the goal
+        const uint64_t gk = bw >> 4;
+        if (((gk + oct(uabs(gx), uabs(gy))) >> q.qshift) != (it >> q.cbits)) return;  // stale
         ++n_exp;
         // window bit (3 * i + j) = cell (x + i - 1, y + j - 1) is in range and open
         uint32_t win = 0;
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
-            int r = x + i - 1;
-            if (r < 0 || r >= q.H) continue;
-            uint64_t wc = (y & 63) == 0 ? whi[i] : wlo[i];  // the word of column y
-            if (y > 0 && ((wlo[i] >> ((y - 1) & 63)) & 1ull)) win |= 1u << (3 * i);
-            if ((wc >> (y & 63)) & 1ull) win |= 1u << (3 * i + 1);
-            if (y + 1 < q.W && ((whi[i] >> ((y + 1) & 63)) & 1ull)) win |= 1u << (3 * i + 2);
+            if ((uint32_t)(x + i - 1) >= (uint32_t)q.H) continue;
+            const uint64_t wc = (y & 63) == 0 ? whi[i] : wlo[i];  // the word of column y
+            win |= (uint32_t)(y > 0 && ((wlo[i] >> ((y - 1) & 63)) & 1ull)) << (3 * i);
+            win |= (uint32_t)((wc >> (y & 63)) & 1ull) << (3 * i + 1);
+            win |= (uint32_t)(y + 1 < q.W && ((whi[i] >> ((y + 1) & 63)) & 1ull)) << (3 * i + 2);
         }
         if (!((win >> 4) & 1u)) return;  // source-cell mask rule
-        // read the neighbours' best words, then relax with an atomic only where the new
-        // cost is lower than what was read (best words only ever decrease, so a skipped
-        // atomic could not have succeeded); most successors on a grid are duplicates
+        // legal moves: in range, and open unless the goal (a blocked cell is a dead end)
+        const int64_t Wl = q.W;
         uint64_t w[8];
         uint32_t live = 0;
 #pragma unroll
         for (int d = 0; d < 8; ++d) {
-            int nx = x + kDX[d], ny = y + kDY[d];
-            if (nx < 0 || nx >= q.H || ny < 0 || ny >= q.W) continue;
-            bool is_goal = (uint64_t)nx * q.W + ny == goal_id;
-            if (!is_goal && !((win >> (3 * (kDX[d] + 1) + (kDY[d] + 1))) & 1u)) continue;  // blocked: a dead end
+            const int nx = x + kDX[d], ny = y + kDY[d];
+            if ((uint32_t)nx >= (uint32_t)q.H || (uint32_t)ny >= (uint32_t)q.W) continue;
+            const bool is_goal = gx + kDX[d] == 0 && gy + kDY[d] == 0;
+            if (!is_goal && !((win >> (3 * (kDX[d] + 1) + (kDY[d] + 1))) & 1u)) continue;
             ++n_gen;
-            uint64_t ng = gk + (d < 4 ? KEY_ONE : KEY_SQRT2);
-            if (ng + octile_key(nx, ny, q.ex, q.ey) >= KEY_LIMIT) {
+            const uint64_t ng = gk + (d < 4 ? KEY_ONE : KEY_SQRT2);
+            if (ng + oct(uabs(gx + kDX[d]), uabs(gy + kDY[d])) >= KEY_LIMIT) {
                 err |= FLAG_RANGE;
                 continue;
             }
             live |= 1u << d;
-            w[d] = sc.load(&q.best[(uint64_t)nx * q.W + ny]);
+            w[d] = sc.load(bp + (kDX[d] * Wl + kDY[d]));
         }
+        // read, then relax with an atomic only where the new cost is lower than what was read
+        // (best words only ever decrease, so a skipped atomic could not have succeeded)
 #pragma unroll
         for (int d = 0; d < 8; ++d) {
-            uint64_t ng = gk + (d < 4 ? KEY_ONE : KEY_SQRT2);
+            const uint64_t ng = gk + (d < 4 ? KEY_ONE : KEY_SQRT2);
             if (!((live >> d) & 1u) || (w[d] >> 4) <= ng) continue;
-            int nx = x + kDX[d], ny = y + kDY[d];
-            uint64_t old = sc.fmin(&q.best[(uint64_t)nx * q.W + ny], (ng << 4) | (uint64_t)d);
+            const uint64_t old = sc.fmin(bp + (kDX[d] * Wl + kDY[d]), (ng << 4) | (uint64_t)d);
             if ((old >> 4) <= ng) continue;
             ++n_ins;
-            if ((uint64_t)nx * q.W + ny == goal_id) {
+            if (gx + kDX[d] == 0 && gy + kDY[d] == 0) {
                 goalmin = goalmin < ng ? goalmin : ng;
             } else {
-                s[d] = item(ng + octile_key(nx, ny, q.ex, q.ey), nx, ny);
+                const uint64_t nf = ng + oct(uabs(gx + kDX[d]), uabs(gy + kDY[d]));
+                s[d] = ((nf >> q.qshift) << q.cbits) | ((uint64_t)(uint32_t)(x + kDX[d]) << q.ybits) |
+                       (uint64_t)(uint32_t)(y + kDY[d]);
                 ++cnt;
             }
         }
@@ -390,7 +401,8 @@ struct PuzzleProb {
         goal_b = goal_board(n);
     }
     __device__ bool trivial() const { return q.start_state == goal_b; }
-    __device__ __forceinline__ uint64_t find_or_insert(const Scope &sc, uint64_t b, uint32_t &err) const {
+    template <class SC>
+    __device__ __forceinline__ uint64_t find_or_insert(const SC &sc, uint64_t b, uint32_t &err) const {
         uint64_t slot = mix64(b) & hmask;
         for (int i = 0; i < MAXPROBE; ++i) {
             uint64_t k = sc.load(&q.best[slot]);
@@ -414,7 +426,8 @@ struct PuzzleProb {
     __device__ bool goal_done(uint64_t goal, uint64_t lb) const {
         return goal != SENT && (goal >> 32) <= (lb >> 56);
     }
-    __device__ __forceinline__ void expand(const Scope &sc, uint64_t it, uint64_t (&s)[MAXS], int &cnt,
+    template <class SC>
+    __device__ __forceinline__ void expand(const SC &sc, uint64_t it, uint64_t (&s)[MAXS], int &cnt,
                                            uint64_t &goalmin, uint32_t &err, int64_t &n_exp, int64_t &n_gen,
                                            int64_t &n_ins) const {
         uint64_t slot = it & ((1ull << 48) - 1);
@@ -699,6 +712,9 @@ __device__ void group_loop(const QueryDesc *qs, QueryResult *rs, const Team &tea
     }
 }
 
+// Keys of one round's insert buffer half that fit in LDS: 16 waves x 64 pops x 8 moves.
+constexpr int kLdsSegCap = kBlockWaves * 64 * 8;
+
 // Body of k_teams for one block: find this block's class and group from the launch
 // shape, then run the group's query loop.
 __device__ inline void group_block(const QueryDesc *qs, QueryResult *rs, const LaunchShape &shape,
@@ -716,13 +732,19 @@ __device__ inline void group_block(const QueryDesc *qs, QueryResult *rs, const L
     group += rel / bpg;
     const int segcap = shape.segcap[cls];
     seg_off += (size_t)(rel / bpg) * 2 * (size_t)segcap;
+    // A one-block team keeps its round slots and insert buffer in LDS: the appends, the
+    // slot atomics and the buffer reads of a round never leave the CU. (The accesses go
+    // through generic pointers, so the same code serves LDS and global memory.)
     __shared__ RoundSlot view;
+    __shared__ RoundSlot lds_slot[2];
+    __shared__ uint64_t lds_seg[2 * kLdsSegCap];
     TeamVars *tv = tvs + group;
     TeamMem m;
-    m.slot = tv->slot;
+    const bool in_lds = bpg == 1 && segcap <= kLdsSegCap;
+    m.slot = in_lds ? lds_slot : tv->slot;
     m.view = &view;
     m.stats = &tv->n_pop;
-    m.seg = segs + seg_off;
+    m.seg = in_lds ? lds_seg : segs + seg_off;
     m.segcap = segcap;
     GroupTeam team(m, tv, bpg, rel % bpg);
     group_loop(qs, rs, team, tv, qhead + cls, shape.nq[cls], shape.qoff[cls], shape.budget_ticks, pdb3, pdb4);
