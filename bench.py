@@ -274,7 +274,7 @@ def cpu_baseline_mixed(specs, budget_s=25.0, threads=None):
 
 def run_serve_mixed(args):
     import uastar_amd as U
-    ctx = U.Context(0)
+    ctx = U.Context(0, graphs=args.graph)
     specs = mixed_stream(args.steps + args.warmup)
     materialize(ctx, specs)
     warm, specs = specs[: args.warmup], specs[args.warmup:]
@@ -307,10 +307,10 @@ def run_serve_mixed(args):
     rng = np.random.default_rng(5)
     arrivals = np.cumsum(rng.exponential(1e3 / rate, len(specs))).tolist()
     arrivals = [a - arrivals[0] for a in arrivals]
-    st0 = ctx.serve_stats()
+    st0, ls0 = ctx.serve_stats(), ctx.launch_stats()
     say(f"serve pass: Poisson arrivals at {rate:.1f} queries/s ({args.load:g} x plain capacity)")
     sres, slat, span, step_wall_ms = serve_pass(ctx, specs, arrivals)
-    st1 = ctx.serve_stats()
+    st1, ls1 = ctx.serve_stats(), ctx.launch_stats()
     serve_qps = len(specs) / (span / 1e3)
     plat = fifo_latency(arrivals, svc)
     mismatch = sum(answer(a, s) != answer(b, s) for a, b, s in zip(pres, sres, specs))
@@ -351,6 +351,13 @@ def run_serve_mixed(args):
         # result copies, launch, retirement), and per plain query outside its kernel
         "host_overhead_ms": {"per_serve_launch": (step_wall_ms - kms) / max(1, launches),
                              "per_plain_query": (sum(svc) - sum(r["kernel_ms"] for r in pres)) / len(pres)},
+        # the serving launches of the pass: replayed from a graph or issued call by call,
+        # and the host time of each launch outside its kernel (staging, copies, launch, wait)
+        "launch": {"graph_launches": ls1["graph_launches"] - ls0["graph_launches"],
+                   "direct_launches": ls1["direct_launches"] - ls0["direct_launches"],
+                   "host_ms_per_launch": ((ls1["wall_ms"] - ls0["wall_ms"]) - (ls1["kernel_ms"] - ls0["kernel_ms"]))
+                   / max(1, launches),
+                   "graph_error": ls1["graph_error"]},
         "tuning": tuning,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
@@ -403,6 +410,8 @@ def main():
     ap.add_argument("--load", type=float, default=20.0, help="arrival rate as a multiple of plain capacity")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default settings)")
+    ap.add_argument("--graph", action="store_true",
+                    help="replay each search launch from a hipGraph instead of issuing it call by call")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (rocprofv3 counter calibration)")
     args = ap.parse_args()
@@ -421,7 +430,7 @@ def main():
     if args.workload == "serve-mixed":
         print(json.dumps(run_serve_mixed(args)))
         return
-    ctx = U.Context(0)
+    ctx = U.Context(0, graphs=args.graph)
 
     specs = grid1000_specs(args.steps)
     warm = grid1000_specs(args.warmup, seed0=1_000_000)

@@ -48,8 +48,12 @@ typedef struct {
     int32_t hash_log2;         /* puzzle hash table size per query; 0 = tuned */
     uint64_t mem_budget;       /* device bytes the batch/serve planner may use; 0 = 80-85% of free */
     int32_t puzzle_team_waves; /* waves (= heaps) per puzzle query in batch/serve (multiple of 16); 0 = tuned */
-    int32_t reserved;
+    int32_t flags;             /* UA_FLAG_* bits; 0 = defaults */
 } ua_config;
+
+/* ua_config.flags: replay each search launch (copy in, cooperative kernel, copy out)
+ * from an instantiated hipGraph instead of issuing it call by call */
+#define UA_FLAG_GRAPH 1
 
 typedef struct {
     int32_t status;
@@ -134,6 +138,17 @@ int ua_serve_drain(ua_ctx *ctx);
  * launch in microseconds and the queries widened so far (long-running 15-puzzles move
  * to a team of a quarter of the device) */
 int ua_serve_stats(ua_ctx *ctx, int64_t *launches, double *kernel_ms, int32_t *budget_us, int64_t *widened);
+
+/* ---- launch statistics (no reference counterpart: uAstar launches its kernels one by
+ * one from a host loop, pathway/GPU-solver.cu:142-290). Every search launch of the
+ * context so far -- plain solves, batch sub-batches, serving steps: how many were
+ * replayed from a graph and how many issued call by call, their summed host wall time
+ * (staging, copies, launch and wait) and their summed kernel time (HIP events). */
+int ua_launch_stats(ua_ctx *ctx, int64_t *graph_launches, int64_t *direct_launches, double *wall_ms,
+                    double *kernel_ms);
+/* why a context created with UA_FLAG_GRAPH stopped replaying launches from graphs
+ * ("" while it replays them, and without the flag) */
+const char *ua_launch_graph_error(ua_ctx *ctx);
 
 /* ---- autotuning (no reference counterpart: uAstar fixes its launch shape and table
  * sizes at compile time, pathway/GPU-solver.hpp:5-17 and puzzle/GPU-solver.cuh:12-22).

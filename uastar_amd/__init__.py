@@ -37,7 +37,10 @@ class UAError(RuntimeError):
 class Config(C.Structure):
     _fields_ = [("team_waves", C.c_int32), ("plain_heaps", C.c_int32), ("max_rounds", C.c_int32),
                 ("hash_log2", C.c_int32), ("mem_budget", C.c_uint64), ("puzzle_team_waves", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("flags", C.c_int32)]
+
+
+FLAG_GRAPH = 1
 
 
 class Result(C.Structure):
@@ -104,6 +107,9 @@ def lib():
             "ua_serve_drain": (C.c_int, [vp]),
             "ua_serve_stats": (C.c_int, [vp, C.POINTER(i64), C.POINTER(C.c_double), C.POINTER(i32),
                                          C.POINTER(i64)]),
+            "ua_launch_stats": (C.c_int, [vp, C.POINTER(i64), C.POINTER(i64), C.POINTER(C.c_double),
+                                          C.POINTER(C.c_double)]),
+            "ua_launch_graph_error": (C.c_char_p, [vp]),
             "ua_debug_heap_selftest": (C.c_int, [vp, vp, C.c_int, vp, i64, C.c_int, vp, i64,
                                                  C.POINTER(i32)]),
             "ua_autotune": (C.c_int, [vp, C.POINTER(Query), i64, C.POINTER(Tuning)]),
@@ -190,8 +196,9 @@ class Context:
     """One device context (ua_ctx)."""
 
     def __init__(self, ordinal=0, team_waves=0, plain_heaps=0, max_rounds=0, hash_log2=0, mem_budget=0,
-                 puzzle_team_waves=0):
-        self.cfg = Config(team_waves, plain_heaps, max_rounds, hash_log2, mem_budget, puzzle_team_waves, 0)
+                 puzzle_team_waves=0, graphs=False):
+        flags = FLAG_GRAPH if graphs else 0
+        self.cfg = Config(team_waves, plain_heaps, max_rounds, hash_log2, mem_budget, puzzle_team_waves, flags)
         h = C.c_void_p()
         rc = lib().ua_create(ordinal, C.byref(self.cfg), C.byref(h))
         if rc:
@@ -326,6 +333,13 @@ class Context:
         n, ms, rb, wd = C.c_int64(), C.c_double(), C.c_int32(), C.c_int64()
         self._check(lib().ua_serve_stats(self.h, C.byref(n), C.byref(ms), C.byref(rb), C.byref(wd)))
         return {"launches": n.value, "kernel_ms": ms.value, "budget_us": rb.value, "widened": wd.value}
+
+    def launch_stats(self):
+        """Every search launch so far: graph replays, direct launches, host wall and kernel ms."""
+        g, d, w, k = C.c_int64(), C.c_int64(), C.c_double(), C.c_double()
+        self._check(lib().ua_launch_stats(self.h, C.byref(g), C.byref(d), C.byref(w), C.byref(k)))
+        return {"graph_launches": g.value, "direct_launches": d.value, "wall_ms": w.value, "kernel_ms": k.value,
+                "graph_error": lib().ua_launch_graph_error(self.h).decode(errors="replace")}
 
     # ---- autotuning
     def autotune(self, queries) -> dict:

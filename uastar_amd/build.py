@@ -23,7 +23,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-I", os.path.join(ROOT, "include"), "-Wall", "-Wno-unused-function"]
 
-LIB_SOURCES = ["kernels.hip", "runtime.cpp", "serve.cpp", "tune.cpp"]
+LIB_SOURCES = ["kernels.hip", "runtime.cpp", "step.cpp", "serve.cpp", "tune.cpp"]
 CLI_SOURCES = ["cli/main.cpp", "cli/pathway.cpp", "cli/puzzle.cpp", "cli/cpu_solvers.cpp"]
 
 

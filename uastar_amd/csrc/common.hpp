@@ -138,6 +138,16 @@ UA_HD inline int shape_blocks(const LaunchShape &s) {
     return b;
 }
 
+// The I/O block of one k_teams launch, in device memory and copied there from the host
+// in one transfer before the launch: this header, the groups' TeamVars (zero), the
+// launch's query descriptors and their results. A query the launch does not reach
+// keeps the result the host put there.
+struct StepHeader {
+    LaunchShape shape;
+    uint32_t qhead[kClasses];             // next query of each class (zero at launch)
+    uint32_t tv_off, qd_off, qr_off, nq;  // byte offsets of TeamVars[], QueryDesc[], QueryResult[]
+};
+
 // Resume record of a suspended query: header, then per heap its root batch,
 // partial batch and (n, pc). Written when a serving launch's round budget runs out.
 constexpr int RESUME_HEADER = 8;   // valid, round, best_goal, pops, expanded, generated, inserted, spare

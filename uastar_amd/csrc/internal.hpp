@@ -70,6 +70,21 @@ struct Tuning {
 };
 }  // namespace uart
 
+namespace uart {
+// An instantiated graph of one k_teams launch (step.cpp): H2D copy of the I/O block,
+// timing event, the cooperative search kernel, timing event, D2H copy of the results.
+// Rebuilt when an address or the block count it was built for changes.
+struct StepGraph {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t x = nullptr;
+    hipGraphNode_t h2d = nullptr, kern = nullptr, d2h = nullptr;
+    void *io = nullptr, *host = nullptr, *segs = nullptr;
+    const void *pdb3 = nullptr, *pdb4 = nullptr;
+    int mode = -1, blocks = 0;
+    void release();
+};
+}  // namespace uart
+
 struct ua_ctx {
     int dev = 0;
     hipStream_t st = nullptr;
@@ -82,13 +97,23 @@ struct ua_ctx {
     size_t total_mem = 0;
     uart::DevBuf pdb[5];
     uart::DevBuf d_added;
+    // launches of k_teams (step.cpp): the I/O block, the groups' insert buffers, the
+    // pinned staging copy of the I/O block and one graph per mode
+    uart::DevBuf io, segs;
+    char *h_io = nullptr;
+    size_t h_io_n = 0;
+    uart::StepGraph graph[8];        // by (mode, block count); reused round robin
+    int graph_next = 0;
+    bool graphs = false;             // replay launches from graphs (UA_FLAG_GRAPH)
+    std::string graph_err;           // why graphs were switched off, if they were
+    int64_t graph_launches = 0, direct_launches = 0;
+    double step_wall_ms = 0, step_kernel_ms = 0;
     // plain solve
-    uart::DevBuf best, vals, heap, seg, tv, qd, qr, qh, pathbuf;
+    uart::DevBuf best, vals, heap, pathbuf;
     ua::QueryDesc last_q{};
     ua::QueryResult last_r{};
     bool last_valid = false;
     // batch
-    uart::DevBuf b_qd, b_qr, b_head, b_tv, b_seg;
     std::vector<uart::DevBuf> b_arenas;        // one per admitted sub-batch, kept for ua_batch_path
     std::vector<int> b_arena_of;         // arena index of each query, -1 = state released
     std::vector<ua::QueryDesc> b_desc;
@@ -136,6 +161,12 @@ int class_of(ua_ctx *c, int kind, int64_t cells);
 int plain_heaps(ua_ctx *c, int kind);
 int batch_hash_log2(ua_ctx *c, int n);
 int plain_hash_log2(ua_ctx *c, int n);
+// One k_teams launch of `blocks` blocks (step.cpp): the shape, the nq descriptors and
+// the results reported for queries the launch does not reach (init, or zeros) go to the
+// device in one copy; the results come back into out. Sets c->last_kernel_ms.
+int run_step(ua_ctx *c, int mode, int blocks, const LaunchShape &shape, const QueryDesc *descs,
+             const QueryResult *init, int nq, QueryResult *out);
+void release_steps(ua_ctx *c);
 // mode: 1 = a ua_batch_solve sub-batch, 2 = a serving step (names the kernel in profiles).
 // budget_ticks > 0: each group stops after that much device time (serving); `init`, if
 // given, is the result reported for a query the launch does not reach.

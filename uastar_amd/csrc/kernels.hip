@@ -14,12 +14,16 @@ namespace ua {
 // from a counter. Plain solve: one group of every block and one query. Batch / serve:
 // one-block teams for grids (workgroup scope), multi-block groups for 15-puzzles. MODE
 // only names the caller (0 plain, 1 batch, 2 serving step), so that profiles report
-// the three uses as separate kernels.
+// the three uses as separate kernels. The shape, the counters, the TeamVars and the
+// queries come from the launch's I/O block (StepHeader), so the arguments of a launch
+// stay the same from step to step; blocks past the shape's own return at once.
 template <int MODE>
-__global__ __launch_bounds__(1024) void k_teams(const QueryDesc *qs, QueryResult *rs, LaunchShape shape,
-                                                uint32_t *qhead, TeamVars *tvs, uint64_t *segs,
-                                                const uint8_t *pdb3, const uint8_t *pdb4) {
-    group_block(qs, rs, shape, qhead, tvs, segs, pdb3, pdb4);
+__global__ __launch_bounds__(1024) void k_teams(char *io, uint64_t *segs, const uint8_t *pdb3,
+                                                const uint8_t *pdb4) {
+    StepHeader *h = reinterpret_cast<StepHeader *>(io);
+    if ((int)blockIdx.x >= shape_blocks(h->shape)) return;
+    group_block(reinterpret_cast<const QueryDesc *>(io + h->qd_off), reinterpret_cast<QueryResult *>(io + h->qr_off),
+                h->shape, h->qhead, reinterpret_cast<TeamVars *>(io + h->tv_off), segs, pdb3, pdb4);
 }
 
 // ------------------------------------------------------------------ path walks
@@ -235,15 +239,14 @@ hipError_t teams_max_blocks(int *per_cu) {
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_teams<0>, kBlockWaves * 64, 0);
 }
 
-hipError_t launch_teams(const QueryDesc *qs, QueryResult *rs, const LaunchShape &shape, uint32_t *qhead,
-                        TeamVars *tvs, uint64_t *segs, const uint8_t *pdb3, const uint8_t *pdb4, int mode,
+const void *teams_fn(int mode) {
+    return mode == 2 ? (const void *)k_teams<2> : mode == 1 ? (const void *)k_teams<1> : (const void *)k_teams<0>;
+}
+
+hipError_t launch_teams(char *io, int blocks, uint64_t *segs, const uint8_t *pdb3, const uint8_t *pdb4, int mode,
                         hipStream_t st) {
-    int blocks = shape_blocks(shape);
-    LaunchShape sh = shape;
-    void *args[] = {(void *)&qs, (void *)&rs, (void *)&sh, (void *)&qhead, (void *)&tvs, (void *)&segs,
-                    (void *)&pdb3, (void *)&pdb4};
-    const void *fn = mode == 2 ? (const void *)k_teams<2> : mode == 1 ? (const void *)k_teams<1> : (const void *)k_teams<0>;
-    return hipLaunchCooperativeKernel(fn, dim3(blocks), dim3(kBlockWaves * 64), args, 0, st);
+    void *args[] = {(void *)&io, (void *)&segs, (void *)&pdb3, (void *)&pdb4};
+    return hipLaunchCooperativeKernel(teams_fn(mode), dim3(blocks), dim3(kBlockWaves * 64), args, 0, st);
 }
 
 hipError_t launch_grid_path(const uint64_t *best, int H, int W, int sx, int sy, int ex, int ey,

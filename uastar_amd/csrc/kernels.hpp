@@ -8,11 +8,13 @@ namespace ua {
 hipError_t launch_grid_gen(uint64_t *bm, int H, int W, int wpr, uint64_t seed, uint64_t thr, int sx,
                            int sy, int ex, int ey, hipStream_t st);
 hipError_t launch_grid_pack(const uint8_t *bytes, uint64_t *bm, int H, int W, int wpr, hipStream_t st);
-// The search kernel (k_teams): shape_blocks(shape) blocks of kBlockWaves wavefronts,
-// cooperative. mode names the caller in profiles: 0 plain solve, 1 batch, 2 serving step.
+// The search kernel (k_teams): `blocks` blocks of kBlockWaves wavefronts, cooperative,
+// over the I/O block io (StepHeader, then TeamVars, descriptors and results). mode
+// names the caller in profiles: 0 plain solve, 1 batch, 2 serving step. teams_fn is the
+// kernel of a mode, for a graph's kernel node (arguments io, segs, pdb3, pdb4).
 hipError_t teams_max_blocks(int *per_cu);
-hipError_t launch_teams(const QueryDesc *qs, QueryResult *rs, const LaunchShape &shape, uint32_t *qhead,
-                        TeamVars *tvs, uint64_t *segs, const uint8_t *pdb3, const uint8_t *pdb4, int mode,
+const void *teams_fn(int mode);
+hipError_t launch_teams(char *io, int blocks, uint64_t *segs, const uint8_t *pdb3, const uint8_t *pdb4, int mode,
                         hipStream_t st);
 hipError_t launch_grid_path(const uint64_t *best, int H, int W, int sx, int sy, int ex, int ey,
                             int32_t *ids, int64_t cap, int64_t *len, hipStream_t st);

@@ -200,10 +200,6 @@ static int run_wide(ua_ctx *c, QueryDesc &q, const Plan &p) {
     HIPCHK(c, c->best.ensure(p.best_bytes));
     if (p.vals_bytes) HIPCHK(c, c->vals.ensure(p.vals_bytes));
     HIPCHK(c, c->heap.ensure(p.heap_bytes));
-    HIPCHK(c, c->seg.ensure((size_t)2 * segcap * 8));
-    HIPCHK(c, c->tv.ensure(sizeof(TeamVars)));
-    HIPCHK(c, c->qd.ensure(sizeof(QueryDesc)));
-    HIPCHK(c, c->qr.ensure(sizeof(QueryResult)));
     q.best = c->best.as<uint64_t>();
     q.vals = c->vals.as<uint32_t>();
     q.heap = c->heap.as<uint64_t>();
@@ -213,27 +209,15 @@ static int run_wide(ua_ctx *c, QueryDesc &q, const Plan &p) {
         HIPCHK(c, hipMemsetAsync(q.best, 0, p.best_bytes, c->st));
         HIPCHK(c, hipMemsetAsync(q.vals, 0xFF, p.vals_bytes, c->st));
     }
-    HIPCHK(c, hipMemsetAsync(c->tv.p, 0, sizeof(TeamVars), c->st));
-    HIPCHK(c, hipMemcpyAsync(c->qd.p, &q, sizeof q, hipMemcpyHostToDevice, c->st));
-    HIPCHK(c, c->qh.ensure(64));
-    HIPCHK(c, hipMemsetAsync(c->qh.p, 0, 64, c->st));
-    HIPCHK(c, hipEventRecord(c->ev0, c->st));
     LaunchShape shape{};
     shape.groups[0] = 1;
     shape.bpg[0] = blocks;
     shape.nq[0] = 1;
     shape.segcap[0] = segcap;
     for (int k = 1; k < kClasses; ++k) shape.bpg[k] = 1;
-    HIPCHK(c, launch_teams(c->qd.as<QueryDesc>(), c->qr.as<QueryResult>(), shape, c->qh.as<uint32_t>(),
-                           c->tv.as<TeamVars>(), c->seg.as<uint64_t>(), c->pdb[3].as<uint8_t>(),
-                           c->pdb[4].as<uint8_t>(), 0, c->st));
-    HIPCHK(c, hipEventRecord(c->ev1, c->st));
     QueryResult r{};
-    HIPCHK(c, hipMemcpyAsync(&r, c->qr.p, sizeof r, hipMemcpyDeviceToHost, c->st));
-    HIPCHK(c, hipStreamSynchronize(c->st));
-    float kms = 0;
-    HIPCHK(c, hipEventElapsedTime(&kms, c->ev0, c->ev1));
-    c->last_kernel_ms = kms;
+    int rc = run_step(c, 0, blocks, shape, &q, nullptr, 1, &r);
+    if (rc) return rc;
     c->last_q = q;
     c->last_r = r;
     c->last_valid = true;
@@ -259,6 +243,7 @@ extern "C" int ua_create(int ordinal, const ua_config *cfg, ua_ctx **out) {
     ua_ctx *c = new ua_ctx();
     c->dev = ordinal;
     if (cfg) c->cfg = *cfg;
+    c->graphs = (c->cfg.flags & UA_FLAG_GRAPH) != 0;
     bool ok = hipSetDevice(ordinal) == hipSuccess &&
               hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) == hipSuccess &&
               hipEventCreate(&c->ev0) == hipSuccess && hipEventCreate(&c->ev1) == hipSuccess;
@@ -282,9 +267,8 @@ extern "C" void ua_destroy(ua_ctx *c) {
     (void)hipStreamSynchronize(c->st);
     ua_serve_release(c);
     for (auto &b : c->pdb) b.release();
-    for (DevBuf *b : {&c->d_added, &c->best, &c->vals, &c->heap, &c->seg, &c->tv, &c->qd, &c->qr, &c->qh,
-                      &c->pathbuf, &c->b_qd, &c->b_qr, &c->b_head, &c->b_tv, &c->b_seg})
-        b->release();
+    release_steps(c);
+    for (DevBuf *b : {&c->d_added, &c->best, &c->vals, &c->heap, &c->pathbuf}) b->release();
     for (auto &b : c->b_arenas) b.release();
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -556,41 +540,19 @@ int run_batch(ua_ctx *c, std::vector<QueryDesc> &descs, std::vector<QueryResult>
         --shape.groups[big];
         used -= shape.bpg[big];
     }
-    int groups = 0;
-    size_t seg_words = 0;
-    for (int k = 0; k < kClasses; ++k) {
-        groups += shape.groups[k];
-        seg_words += (size_t)shape.groups[k] * 2 * shape.segcap[k];
-    }
     shape.budget_ticks = budget_ticks;
     std::vector<QueryDesc> sorted(nq);
     for (int k = 0; k < nq; ++k) sorted[k] = descs[order[k]];
-    HIPCHK(c, c->b_qd.ensure(sizeof(QueryDesc) * nq));
-    HIPCHK(c, c->b_qr.ensure(sizeof(QueryResult) * nq));
-    HIPCHK(c, c->b_head.ensure(64));
-    HIPCHK(c, c->b_tv.ensure(sizeof(TeamVars) * groups));
-    HIPCHK(c, c->b_seg.ensure(seg_words * 8));
-    HIPCHK(c, hipMemcpyAsync(c->b_qd.p, sorted.data(), sizeof(QueryDesc) * nq, hipMemcpyHostToDevice, c->st));
     std::vector<QueryResult> sinit;
     if (init) {
         sinit.resize(nq);
         for (int k = 0; k < nq; ++k) sinit[k] = (*init)[order[k]];
-        HIPCHK(c, hipMemcpyAsync(c->b_qr.p, sinit.data(), sizeof(QueryResult) * nq, hipMemcpyHostToDevice, c->st));
     }
-    HIPCHK(c, hipMemsetAsync(c->b_head.p, 0, 64, c->st));
-    HIPCHK(c, hipMemsetAsync(c->b_tv.p, 0, sizeof(TeamVars) * groups, c->st));
-    HIPCHK(c, hipEventRecord(c->ev0, c->st));
-    HIPCHK(c, launch_teams(c->b_qd.as<QueryDesc>(), c->b_qr.as<QueryResult>(), shape, c->b_head.as<uint32_t>(),
-                           c->b_tv.as<TeamVars>(), c->b_seg.as<uint64_t>(), c->pdb[3].as<uint8_t>(),
-                           c->pdb[4].as<uint8_t>(), mode, c->st));
-    HIPCHK(c, hipEventRecord(c->ev1, c->st));
+    // every resident block: the grid of a launch stays the same from step to step
     std::vector<QueryResult> sres(nq);
-    HIPCHK(c, hipMemcpyAsync(sres.data(), c->b_qr.p, sizeof(QueryResult) * nq, hipMemcpyDeviceToHost, c->st));
-    HIPCHK(c, hipStreamSynchronize(c->st));
+    int rc = run_step(c, mode, resident, shape, sorted.data(), init ? sinit.data() : nullptr, nq, sres.data());
+    if (rc) return rc;
     for (int k = 0; k < nq; ++k) res[order[k]] = sres[k];
-    float kms = 0;
-    HIPCHK(c, hipEventElapsedTime(&kms, c->ev0, c->ev1));
-    c->last_kernel_ms = kms;
     return UA_OK;
 }
 

@@ -717,7 +717,7 @@ constexpr int kLdsSegCap = kBlockWaves * 64 * 8;
 
 // Body of k_teams for one block: find this block's class and group from the launch
 // shape, then run the group's query loop.
-__device__ inline void group_block(const QueryDesc *qs, QueryResult *rs, const LaunchShape &shape,
+__device__ __forceinline__ void group_block(const QueryDesc *qs, QueryResult *rs, const LaunchShape &shape,
                                    uint32_t *qhead, TeamVars *tvs, uint64_t *segs, const uint8_t *pdb3,
                                    const uint8_t *pdb4) {
     int rel = (int)blockIdx.x, cls = 0, group = 0;
