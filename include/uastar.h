@@ -95,9 +95,10 @@ int ua_grid_solve(ua_ctx *ctx, const ua_grid *g, int sx, int sy, int ex, int ey,
  * GPUPathwaySolver::getSolution (pathway/GPU-solver.cu:292-316). */
 int ua_grid_path(ua_ctx *ctx, int32_t *ids, int64_t cap, int64_t *len);
 
-/* ---- sliding-tile puzzles, n = 3 or 4. */
+/* ---- sliding-tile puzzles, n = 3, 4 or 5. */
 /* Builds the reference's additive disjoint PDB on the device (puzzle/database.cpp:37-84;
- * partitions of puzzle/CPU-solver.hpp:44-70); replaces the PDB upload of
+ * partitions of puzzle/CPU-solver.hpp:44-104: one 8-tile table for n = 3, 8 + 7 tiles for
+ * n = 4, four 6-tile tables for n = 5); replaces the PDB upload of
  * GPUPuzzleSolver<N>::initialize (puzzle/GPU-solver.cuh:130-160). */
 int ua_puzzle_build_pdb(ua_ctx *ctx, int n);
 /* Copies the device PDB (all partitions concatenated) to the host. */
@@ -119,7 +120,7 @@ typedef struct {
     int32_t n;               /* puzzle side */
     const ua_grid *grid;     /* grid queries */
     int32_t sx, sy, ex, ey;
-    uint8_t tiles[16];       /* puzzle board */
+    uint8_t tiles[25];       /* puzzle board, n*n entries */
 } ua_query;
 int ua_batch_solve(ua_ctx *ctx, const ua_query *qs, int64_t nq, ua_result *out);
 /* path / moves of query i of the last ua_batch_solve (while its state is resident) */

@@ -6,8 +6,8 @@
 
 #include "../../uastar_amd/csrc/search.hpp"
 
-extern "C" int emu_batch(const ua::QueryDesc *qs, ua::QueryResult *rs, int nq, int waves, const uint8_t *pdb3,
-                         const uint8_t *pdb4) {
+template <int SET>
+static int run_batch(const ua::QueryDesc *qs, ua::QueryResult *rs, int nq, int waves) {
     if (waves != 4 && waves != 8 && waves != 16) return -1;
     const int segcap = waves * 64 * 8;
     std::vector<uint64_t> segs((size_t)2 * segcap, 0);
@@ -26,9 +26,27 @@ extern "C" int emu_batch(const ua::QueryDesc *qs, ua::QueryResult *rs, int nq, i
     for (int t = 0; t < waves * 64; ++t)
         th.emplace_back([&, t] {
             emu::tid = (unsigned)t;
-            ua::group_block(qs, rs, shape, qhead, &tv, segs.data(), pdb3, pdb4);
+            ua::group_block<SET>(qs, rs, shape, qhead, &tv, segs.data());
         });
     for (auto &x : th) x.join();
     emu::block = nullptr;
     return 0;
+}
+
+extern "C" int emu_batch(const ua::QueryDesc *qs, ua::QueryResult *rs, int nq, int waves) {
+    return run_batch<ua::SET_MAIN>(qs, rs, nq, waves);
+}
+// the 5x5 problem set (k_teams<., SET_FIVE>)
+extern "C" int emu_batch5(const ua::QueryDesc *qs, ua::QueryResult *rs, int nq, int waves) {
+    return run_batch<ua::SET_FIVE>(qs, rs, nq, waves);
+}
+
+// 5x5 board helpers of the device search, for direct checks against the oracle
+extern "C" uint32_t emu_rank5(uint64_t w0, uint64_t w1, int p) { return ua::rank5(w0, w1, p); }
+extern "C" int emu_pos5(uint64_t w0, uint64_t w1, int t) { return ua::pos5(w0, w1, t); }
+extern "C" int emu_cell5(uint64_t w0, uint64_t w1, int c) { return (int)ua::cell5(w0, w1, c); }
+extern "C" void emu_move5(uint64_t *w0, uint64_t *w1, int bl, int nb) {
+    uint64_t t = ua::cell5(*w0, *w1, nb);
+    ua::xor5(*w0, *w1, bl, t);
+    ua::xor5(*w0, *w1, nb, t);
 }

@@ -4,6 +4,8 @@
     python tests/golden/make_golden.py c3       # 256 random solvable 15-puzzles, seeds 3000..3255
     python tests/golden/make_golden.py c4       # 512 random solvable 15-puzzles, seeds 41000..41511
     python tests/golden/make_golden.py grids    # seeded grid configs (1000^2 seed 1; 16384^2 seed 2)
+    python tests/golden/make_golden.py pdb5     # the four 6-tile 5x5 PDB digests (added to pdb_digests.json)
+    python tests/golden/make_golden.py c5       # 5x5 boards from seeded random walks off the goal
 
 The oracle itself is pinned by tests/test_oracle.py (closed forms, exhaustive BFS,
 textbook Dijkstra / IDA*); these files record its answers at sizes the GPU tests
@@ -28,9 +30,10 @@ from oracle import oracle as O  # noqa: E402
 CACHE = os.path.join(ROOT, "_cache")
 
 
-def cmd_pdb():
-    out = {}
-    for n in (3, 4):
+def cmd_pdb(sizes=(3, 4)):
+    path = os.path.join(HERE, "pdb_digests.json")
+    out = json.load(open(path)) if os.path.exists(path) else {}
+    for n in sizes:
         pdb = O.pdb_full(n, 1, cache_dir=CACHE)
         parts = O.partitions(n)
         off = 0
@@ -42,8 +45,50 @@ def cmd_pdb():
             tabs.append({"tiles": t, "size": int(size), "sha256": hashlib.sha256(tab.tobytes()).hexdigest(),
                          "histogram": np.bincount(tab, minlength=256).tolist()})
         out[str(n)] = tabs
-    with open(os.path.join(HERE, "pdb_digests.json"), "w") as f:
+    with open(path, "w") as f:
         json.dump(out, f, indent=1)
+
+
+def walk_board(n, seed, length):
+    """The board `length` blank moves away from the goal along a seeded walk that never
+    undoes its previous move (moves drawn from the oracle's mix64 stream)."""
+    b = list(range(1, n * n)) + [0]
+    z, last, s = n * n - 1, -1, seed
+    for _ in range(length):
+        while True:
+            s = (s + 0x9E3779B97F4A7C15) & (2**64 - 1)
+            d = O.lib().orc_mix64(s) % 4
+            x, y = z // n + (1, -1, 0, 0)[d], z % n + (0, 0, 1, -1)[d]
+            if 0 <= x < n and 0 <= y < n and (last < 0 or d != (last ^ 1)):
+                break
+        nz = x * n + y
+        b[z], b[nz] = b[nz], b[z]
+        z, last = nz, d
+    return b
+
+
+def cmd_c5():
+    """5x5 boards from walks of 30..120 moves; optimal steps by IDA* with Manhattan
+    distance (textbook) and by A* with the reference's 6-6-6-6 PDB, which must agree."""
+    pdb5 = O.pdb_full(5, 1, cache_dir=CACHE)
+    rows = []
+    for i, length in enumerate([30, 40, 50, 60, 70, 80, 90, 100, 110, 120] * 2):
+        seed = 5000 + i
+        b = walk_board(5, seed, length)
+        t = time.time()
+        steps, moves, exp = O.puzzle_solve(5, np.array(b, np.uint8), None, "ida_manhattan", node_limit=400_000_000)
+        t_ida = time.time() - t
+        if steps < 0:
+            print(seed, length, "IDA* node limit", flush=True)
+            continue
+        t = time.time()
+        s2, _, exp2 = O.puzzle_solve(5, np.array(b, np.uint8), pdb5, "astar", node_limit=20_000_000)
+        assert s2 == steps, (seed, s2, steps)
+        rows.append({"seed": seed, "walk": length, "board": b, "steps": steps, "moves": moves,
+                     "expanded": exp2, "ida_nodes": exp, "cpu_s": round(time.time() - t, 3)})
+        print(seed, length, steps, exp, round(t_ida, 1), exp2, flush=True)
+    with open(os.path.join(HERE, "c5_puzzles.json"), "w") as f:
+        json.dump(rows, f)
 
 
 _pdb4 = None
@@ -94,4 +139,5 @@ def cmd_grids():
 
 
 if __name__ == "__main__":
-    {"pdb": cmd_pdb, "c3": cmd_c3, "c4": cmd_c4, "grids": cmd_grids}[sys.argv[1]]()
+    {"pdb": cmd_pdb, "pdb5": lambda: cmd_pdb((5,)), "c3": cmd_c3, "c4": cmd_c4, "grids": cmd_grids,
+     "c5": cmd_c5}[sys.argv[1]]()

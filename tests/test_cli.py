@@ -103,7 +103,8 @@ def test_unsolvable_puzzle_reports_no_solution(tmp_path):
 @pytest.mark.gpu
 def test_cli_cpu_and_gpu_agree(tmp_path):
     """Full runs (CPU and GPU, uAstar's consistency check) on a random grid, the README
-    grid and an 8-puzzle; a 15-puzzle on the GPU alone against the CPU A* fixture."""
+    grid and an 8-puzzle; a 15-puzzle and a 24-puzzle on the GPU alone against the
+    fixtures. (The CLI runs while this test process holds no device context.)"""
     print("cli: random grid 400x300", flush=True)
     rc, out = run(["--pathway", "-H", "400", "-W", "300", "--input-module", "random", "-b", "30", "-s", "5"],
                   cwd=tmp_path)
@@ -122,4 +123,12 @@ def test_cli_cpu_and_gpu_agree(tmp_path):
     assert f"Optimal steps from GPU: {row['steps']}" in out
     states = [b for b in (tmp_path / "solutionGPU.txt").read_text().split("\n\n") if b.strip()]
     assert [int(v) for v in states[-1].split()] == goal_board(4)
+    assert len(states) == row["steps"] + 1
+    row = [r for r in json.load(open(os.path.join(GOLDEN, "c5_puzzles.json"))) if r["expanded"] < 1_000_000][0]
+    print(f"cli: 24-puzzle seed {row['seed']}", flush=True)
+    rc, out = run(["--puzzle", "-W", "5", "-G"], " ".join(map(str, row["board"])), cwd=tmp_path)
+    assert rc == 0, out
+    assert f"Optimal steps from GPU: {row['steps']}" in out
+    states = [b for b in (tmp_path / "solutionGPU.txt").read_text().split("\n\n") if b.strip()]
+    assert [int(v) for v in states[-1].split()] == goal_board(5)
     assert len(states) == row["steps"] + 1

@@ -24,7 +24,8 @@ class QueryDesc(C.Structure):
     _fields_ = [(n, C.c_int32) for n in ("kind", "n", "H", "W", "sx", "sy", "ex", "ey", "wpr", "ybits", "cbits",
                                           "qshift", "heap_cap", "hash_log2", "max_rounds", "heaps")] + [
         ("bitmap", C.c_void_p), ("best", C.c_void_p), ("vals", C.c_void_p), ("heap", C.c_void_p),
-        ("start_state", C.c_uint64), ("resume", C.c_void_p), ("round_budget", C.c_int32), ("cls", C.c_int32)]
+        ("start_state", C.c_uint64), ("start_state2", C.c_uint64), ("pdb", C.c_void_p), ("resume", C.c_void_p),
+        ("round_budget", C.c_int32), ("cls", C.c_int32)]
 
 RESUME_WORDS_4 = 8 + 4 * 130
 
@@ -44,7 +45,7 @@ def emu():
                                srcs[0], "-o", LIB + ".tmp"])
         os.replace(LIB + ".tmp", LIB)
     L = C.CDLL(LIB)
-    L.emu_batch.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
+    L.emu_batch.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int]
     return L
 
 
@@ -76,7 +77,7 @@ class Batch:
         self.keep += [bm, best, heap]
         self.descs.append(d)
 
-    def add_puzzle(self, n, tiles, hash_log2=17, items=65536):
+    def add_puzzle(self, n, tiles, pdb, hash_log2=17, items=65536):
         keys = np.zeros(1 << hash_log2, np.uint64)
         vals = np.full(1 << hash_log2, 0xFFFFFFFF, np.uint32)
         per_heap = (items + 64 * self.waves - 1) // (64 * self.waves)
@@ -87,15 +88,14 @@ class Batch:
             s |= int(t) << (4 * i)
         d = QueryDesc(kind=2, n=n, hash_log2=hash_log2, heap_cap=cap, max_rounds=self.max_rounds, heaps=self.waves)
         d.best, d.vals, d.heap, d.start_state = keys.ctypes.data, vals.ctypes.data, heap.ctypes.data, s
-        self.keep += [keys, vals, heap]
+        d.pdb = pdb.ctypes.data
+        self.keep += [keys, vals, heap, pdb]
         self.descs.append(d)
 
-    def run(self, lib, pdb3=None, pdb4=None):
+    def run(self, lib):
         arr = (QueryDesc * len(self.descs))(*self.descs)
         res = (QueryResult * len(self.descs))()
-        p3 = pdb3.ctypes.data if pdb3 is not None else None
-        p4 = pdb4.ctypes.data if pdb4 is not None else None
-        assert lib.emu_batch(arr, res, len(self.descs), self.waves, p3, p4) == 0
+        assert lib.emu_batch(arr, res, len(self.descs), self.waves) == 0
         return list(res)
 
 
@@ -126,9 +126,9 @@ def test_emulated_batch_puzzles(emu, oracle):
     exp = []
     for s in range(4):
         t = oracle.puzzle_random(3, 900 + s)
-        b.add_puzzle(3, t)
+        b.add_puzzle(3, t, pdb3)
         exp.append(oracle.puzzle_solve(3, t, pdb3, "astar")[0])
-    res = b.run(emu, pdb3=pdb3)
+    res = b.run(emu)
     for r, e in zip(res, exp):
         assert r.status == 0 and r.found and (r.goal >> 32) == e
 
@@ -148,7 +148,7 @@ def test_emulated_suspend_resume(emu, oracle):
         exp.append(oracle.grid_astar(g, H, W, 0, 0, H - 1, W - 1, "ref"))
     pdb3 = oracle.pdb_full(3)
     t = oracle.puzzle_random(3, 31337)
-    b.add_puzzle(3, t)
+    b.add_puzzle(3, t, pdb3)
     exp_p = oracle.puzzle_solve(3, t, pdb3, "astar")[0]
     recs = [np.zeros(RESUME_WORDS_4, np.uint64) for _ in b.descs]
     for d, r in zip(b.descs, recs):
@@ -160,7 +160,7 @@ def test_emulated_suspend_resume(emu, oracle):
         idx = [i for i, f in enumerate(final) if f is None]
         sub = Batch(waves=4)
         sub.descs = [b.descs[i] for i in idx]
-        res = sub.run(emu, pdb3=pdb3)
+        res = sub.run(emu)
         launches += 1
         for i, r in zip(idx, res):
             if r.status != 2:  # not suspended
@@ -172,3 +172,106 @@ def test_emulated_suspend_resume(emu, oracle):
         if e["found"]:
             assert U.key_to_pair(r.goal) == (e["orth"], e["diag"])
     assert final[3].status == 0 and (final[3].goal >> 32) == exp_p
+
+
+# ---------------------------------------------------------------- 5x5 boards
+
+def pack5(tiles):
+    """PuzzleStorage<5> layout (puzzle/storage.hpp:144-170): cells 0..11 in w0, 12..23 in
+    w1, 5 bits each; cell 24's low 4 bits at w0 bit 60, its bit 4 at w1 bit 60."""
+    t = [int(v) for v in tiles]
+    w0 = sum(t[i] << (5 * i) for i in range(12)) | ((t[24] & 15) << 60)
+    w1 = sum(t[i] << (5 * (i - 12)) for i in range(12, 24)) | ((t[24] & 16) << 56)
+    return w0, w1
+
+
+def short_walk(rng, length):
+    b = list(range(1, 25)) + [0]
+    z, last = 24, -1
+    for _ in range(length):
+        while True:
+            d = int(rng.integers(4))
+            x, y = z // 5 + (1, -1, 0, 0)[d], z % 5 + (0, 0, 1, -1)[d]
+            if 0 <= x < 5 and 0 <= y < 5 and (last < 0 or d != (last ^ 1)):
+                break
+        nz = x * 5 + y
+        b[z], b[nz] = b[nz], b[z]
+        z, last = nz, d
+    return b
+
+
+@pytest.fixture(scope="module")
+def emu5(emu):
+    emu.emu_batch5.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int]
+    emu.emu_rank5.restype = C.c_uint32
+    emu.emu_rank5.argtypes = [C.c_uint64, C.c_uint64, C.c_int]
+    emu.emu_pos5.argtypes = [C.c_uint64, C.c_uint64, C.c_int]
+    emu.emu_cell5.argtypes = [C.c_uint64, C.c_uint64, C.c_int]
+    emu.emu_move5.argtypes = [C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.c_int, C.c_int]
+    return emu
+
+
+def test_board5_cells_positions_ranks(emu5, oracle):
+    """The device's 5x5 board helpers against the oracle: every cell and tile position,
+    the k-permutation rank of each 6-tile partition (database.cpp:140-163) and a blank
+    move, on random boards and the goal."""
+    parts = oracle.partitions(5)
+    rng = np.random.default_rng(55)
+    boards = [list(range(1, 25)) + [0]] + [oracle.puzzle_random(5, 7700 + i).tolist() for i in range(40)]
+    for b in boards:
+        w0, w1 = pack5(b)
+        for c in range(25):
+            assert emu5.emu_cell5(w0, w1, c) == b[c]
+            assert emu5.emu_pos5(w0, w1, b[c]) == c
+        for p, tracked in enumerate(parts):
+            assert emu5.emu_rank5(w0, w1, p) == oracle.pdb_encode(5, tracked, np.array(b, np.uint8))
+        z = b.index(0)
+        for d in range(4):
+            x, y = z // 5 + (1, -1, 0, 0)[d], z % 5 + (0, 0, 1, -1)[d]
+            if not (0 <= x < 5 and 0 <= y < 5):
+                continue
+            nb = x * 5 + y
+            a0, a1 = C.c_uint64(w0), C.c_uint64(w1)
+            emu5.emu_move5(C.byref(a0), C.byref(a1), z, nb)
+            c2 = list(b)
+            c2[z], c2[nb] = c2[nb], c2[z]
+            assert (a0.value, a1.value) == pack5(c2)
+    assert rng is not None
+
+
+def test_emulated_batch_puzzles5(emu5, oracle):
+    """5x5 searches of the SET_FIVE kernel body on short walks off the goal, with the
+    first 6-tile table of the reference's PDB (the other three zero: still admissible);
+    steps equal IDA* with Manhattan distance."""
+    parts = oracle.partitions(5)
+    size = oracle.pdb_size(5, 6)
+    pdb = np.zeros(4 * size, np.uint8)
+    pdb[:size] = oracle.pdb_build(5, parts[0], 1)
+    rng = np.random.default_rng(505)
+    b = Batch(waves=4)
+    exp = []
+    for length in (0, 3, 8, 12, 14):
+        t = short_walk(rng, length)
+        keys = np.zeros(2 << 17, np.uint64)
+        vals = np.full(1 << 17, 0xFFFFFFFF, np.uint32)
+        cap = max((65536 + 64 * 4 - 1) // (64 * 4) + 1, 4)
+        heap = np.zeros(4 * (cap - 1) * 64, np.uint64)
+        d = QueryDesc(kind=2, n=5, hash_log2=17, heap_cap=cap, max_rounds=100_000, heaps=4)
+        d.best, d.vals, d.heap, d.pdb = keys.ctypes.data, vals.ctypes.data, heap.ctypes.data, pdb.ctypes.data
+        d.start_state, d.start_state2 = pack5(t)
+        b.keep += [keys, vals, heap]
+        b.descs.append(d)
+        exp.append(oracle.puzzle_solve(5, np.array(t, np.uint8), None, "ida_manhattan")[0])
+    b.keep.append(pdb)
+    arr = (QueryDesc * len(b.descs))(*b.descs)
+    res = (QueryResult * len(b.descs))()
+    assert emu5.emu_batch5(arr, res, len(b.descs), 4) == 0
+    for r, e in zip(res, exp):
+        assert r.status == 0 and r.found and (r.goal >> 32) == e, (r.status, r.goal >> 32, e)
+    # a 4x4 query sent to the 5x5 kernel fails cleanly
+    b4 = Batch(waves=4)
+    b4.add_puzzle(4, oracle.puzzle_random(4, 1), np.zeros(1, np.uint8))
+    arr = (QueryDesc * 1)(*b4.descs)
+    res = (QueryResult * 1)()
+    assert emu5.emu_batch5(arr, res, 1, 4) == 0
+    assert res[0].status == -1

@@ -209,9 +209,9 @@ def test_eight_puzzle_astar_vs_bfs(oracle, bfs8):
 
 
 def test_small_pattern_pdb_queue_equals_levels(oracle):
-    for tracked in ([1, 2, 3], [5, 9, 13, 14], [15]):
-        a = oracle.pdb_build(4, tracked, 0)
-        b = oracle.pdb_build(4, tracked, 1)
+    for n, tracked in ((4, [1, 2, 3]), (4, [5, 9, 13, 14]), (4, [15]), (5, [2, 1, 6]), (5, [13, 14, 18, 19])):
+        a = oracle.pdb_build(n, tracked, 0)
+        b = oracle.pdb_build(n, tracked, 1)
         assert np.array_equal(a, b)
         assert not (a == 255).any()   # >= 8 free cells: every placement is reachable
 
@@ -260,3 +260,35 @@ def test_fifteen_puzzle_pdb_and_astar(oracle):
                 c = b.copy()
                 c[z], c[x * 4 + y] = c[x * 4 + y], c[z]
                 assert abs(oracle.puzzle_h(4, c, pdb) - h0) == 1
+
+
+@pytest.mark.slow
+def test_twentyfour_puzzle_pdb_and_astar(oracle):
+    """The reference's 6-6-6-6 PDB of the 5x5 board (level BFS) against its invariants,
+    and A* with it against IDA* with Manhattan distance on walks off the goal."""
+    parts = oracle.partitions(5)
+    assert parts == [[3, 4, 5, 10, 15, 20], [2, 1, 6, 11, 16, 21], [7, 8, 9, 12, 17, 22], [13, 14, 18, 19, 23, 24]]
+    pdb = oracle.pdb_full(5, 1, cache_dir="_cache")
+    assert pdb.size == 4 * 127512000
+    assert not (pdb == 255).any()
+    goal = np.array(goal_board(5), np.uint8)
+    assert oracle.puzzle_h(5, goal, pdb) == 0
+    rng = np.random.default_rng(25)
+    for _ in range(8):
+        b = random_walk_board(rng, 5, int(rng.integers(10, 36)))
+        s1, m1, _ = oracle.puzzle_solve(5, b, pdb, "astar")
+        s2, _, _ = oracle.puzzle_solve(5, b, None, "ida_manhattan", node_limit=50_000_000)
+        assert s1 == s2
+        assert replay_moves(5, b, m1) == goal_board(5)
+        h = oracle.puzzle_h(5, b, pdb)
+        assert h <= s1 and (s1 - h) % 2 == 0
+    for _ in range(200):  # consistency: a blank move changes h by exactly one
+        b = oracle.puzzle_random(5, int(rng.integers(1 << 40)))
+        z = int(np.flatnonzero(b == 0)[0])
+        h0 = oracle.puzzle_h(5, b, pdb)
+        for i in range(4):
+            x, y = z // 5 + DX[i], z % 5 + DY[i]
+            if 0 <= x < 5 and 0 <= y < 5:
+                c = b.copy()
+                c[z], c[x * 5 + y] = c[x * 5 + y], c[z]
+                assert abs(oracle.puzzle_h(5, c, pdb) - h0) == 1

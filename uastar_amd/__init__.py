@@ -24,6 +24,7 @@ UA_NO_PATH = 1
 STATUS_NAMES = {0: "OK", 1: "NO_PATH", -1: "INVALID", -2: "OOM", -3: "CAPACITY", -4: "RANGE",
                 -5: "HIP", -6: "NO_DEVICE", -7: "ROUNDS", -8: "NO_PDB", -9: "ABORT"}
 QUERY_GRID = 1
+PDB_BYTES = {3: 362880, 4: 576576000, 5: 510048000}  # the partitions' tables of side n, concatenated
 QUERY_PUZZLE = 2
 SQRT2 = math.sqrt(2.0)
 
@@ -55,7 +56,7 @@ class Result(C.Structure):
 
 class Query(C.Structure):
     _fields_ = [("kind", C.c_int32), ("n", C.c_int32), ("grid", C.c_void_p), ("sx", C.c_int32),
-                ("sy", C.c_int32), ("ex", C.c_int32), ("ey", C.c_int32), ("tiles", C.c_uint8 * 16)]
+                ("sy", C.c_int32), ("ex", C.c_int32), ("ey", C.c_int32), ("tiles", C.c_uint8 * 25)]
 
 
 class Tuning(C.Structure):
@@ -258,7 +259,7 @@ class Context:
         self._check(lib().ua_puzzle_build_pdb(self.h, n))
 
     def puzzle_download_pdb(self, n: int) -> np.ndarray:
-        size = 576576000 if n == 4 else 362880
+        size = PDB_BYTES[n]
         out = np.empty(size, np.uint8)
         self._check(lib().ua_puzzle_download_pdb(self.h, n, out.ctypes.data, size))
         return out

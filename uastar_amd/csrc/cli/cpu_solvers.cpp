@@ -83,9 +83,11 @@ bool cpu_grid_astar(const std::vector<uint8_t> &graph, int H, int W, int sx, int
 
 // ------------------------------------------------------------------ pattern databases
 
-std::vector<std::vector<int>> pdb_partitions(int n) {
+std::vector<std::vector<int>> pdb_partitions(int n) {  // puzzle/CPU-solver.hpp:44-104
     if (n == 3) return {{1, 2, 3, 4, 5, 6, 7, 8}};
-    return {{1, 2, 3, 4, 5, 6, 9, 13}, {7, 8, 10, 11, 12, 14, 15}};  // puzzle/CPU-solver.hpp:52-69
+    if (n == 5)
+        return {{3, 4, 5, 10, 15, 20}, {2, 1, 6, 11, 16, 21}, {7, 8, 9, 12, 17, 22}, {13, 14, 18, 19, 23, 24}};
+    return {{1, 2, 3, 4, 5, 6, 9, 13}, {7, 8, 10, 11, 12, 14, 15}};
 }
 
 size_t pdb_table_size(int n, size_t k) {
@@ -231,11 +233,25 @@ struct PuzzleCtx {
     }
 };
 
-uint64_t pack(const std::vector<uint8_t> &b) {
-    uint64_t s = 0;
-    for (size_t i = 0; i < b.size(); ++i) s |= (uint64_t)b[i] << (4 * i);
+// A board as one 128-bit word, cell i at bits 5i..5i+4 (boards of up to 25 cells).
+using Board = unsigned __int128;
+
+Board pack(const std::vector<uint8_t> &b) {
+    Board s = 0;
+    for (size_t i = 0; i < b.size(); ++i) s |= (Board)b[i] << (5 * i);
     return s;
 }
+
+inline unsigned tile_at(Board s, int cell) { return (unsigned)(s >> (5 * cell)) & 31u; }
+
+struct BoardHash {
+    size_t operator()(Board s) const {
+        uint64_t lo = (uint64_t)s, hi = (uint64_t)(s >> 64);
+        lo ^= hi * 0x9E3779B97F4A7C15ull;
+        lo ^= lo >> 29;
+        return (size_t)(lo * 0xBF58476D1CE4E5B9ull);
+    }
+};
 
 }  // namespace
 
@@ -269,17 +285,17 @@ bool cpu_puzzle_astar(int n, const std::vector<uint8_t> &board, const std::vecto
     std::vector<uint8_t> goal(cells);
     for (int i = 0; i < cells - 1; ++i) goal[i] = (uint8_t)(i + 1);
     goal[cells - 1] = 0;
-    const uint64_t goal_s = pack(goal), start = pack(board);
-    auto cells_of = [&](uint64_t s, uint8_t *cell_of_tile) {
-        for (int i = 0; i < cells; ++i) cell_of_tile[(s >> (4 * i)) & 15] = (uint8_t)i;
+    const Board goal_s = pack(goal), start = pack(board);
+    auto cells_of = [&](Board s, uint8_t *cell_of_tile) {
+        for (int i = 0; i < cells; ++i) cell_of_tile[tile_at(s, i)] = (uint8_t)i;
     };
     struct Rec {
         uint8_t g, dir;  // dir: move that reached the state, 15 = start
     };
-    std::unordered_map<uint64_t, Rec> seen;
-    using Item = std::tuple<int, int, uint64_t>;  // (f, -g, state): deeper first on ties
+    std::unordered_map<Board, Rec, BoardHash> seen;
+    using Item = std::tuple<int, int, Board>;  // (f, -g, state): deeper first on ties
     std::priority_queue<Item, std::vector<Item>, std::greater<Item>> open;
-    uint8_t ct[16];
+    uint8_t ct[25];
     cells_of(start, ct);
     seen[start] = {0, 15};
     open.push({ctx.h(ct), 0, start});
@@ -291,14 +307,14 @@ bool cpu_puzzle_astar(int n, const std::vector<uint8_t> &board, const std::vecto
         if (r.g != g) continue;
         if (s == goal_s) {
             std::vector<int> rev;
-            uint64_t cur = s;
+            Board cur = s;
             while (seen[cur].dir != 15) {  // undo the moves back to the start
                 int d = seen[cur].dir;
                 cells_of(cur, ct);
                 int bl = ct[0], bx = bl / n - DX[d], by = bl % n - DY[d];
                 int from = bx * n + by;  // where the blank was before move d
-                uint64_t t = (cur >> (4 * from)) & 15;
-                cur = cur - (t << (4 * from)) + (t << (4 * bl));
+                Board t = tile_at(cur, from);
+                cur = cur - (t << (5 * from)) + (t << (5 * bl));
                 rev.push_back(d);
             }
             *steps = g;
@@ -312,13 +328,13 @@ bool cpu_puzzle_astar(int n, const std::vector<uint8_t> &board, const std::vecto
             int nx = bx + DX[d], ny = by + DY[d];
             if (nx < 0 || nx >= n || ny < 0 || ny >= n) continue;
             int nb = nx * n + ny;
-            uint64_t t = (s >> (4 * nb)) & 15;
-            uint64_t ns = s + (t << (4 * bl)) - (t << (4 * nb));
+            unsigned t = tile_at(s, nb);
+            Board ns = s + ((Board)t << (5 * bl)) - ((Board)t << (5 * nb));
             auto it = seen.find(ns);
             if (it != seen.end() && it->second.g <= g + 1) continue;
             seen[ns] = {(uint8_t)(g + 1), (uint8_t)d};
-            uint8_t nct[16];
-            std::copy(ct, ct + 16, nct);
+            uint8_t nct[25];
+            std::copy(ct, ct + 25, nct);
             nct[0] = (uint8_t)nb;
             nct[t] = (uint8_t)bl;
             open.push({g + 1 + ctx.h(nct), -(g + 1), ns});

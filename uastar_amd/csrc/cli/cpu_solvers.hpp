@@ -19,7 +19,7 @@ struct GridCost {
 bool cpu_grid_astar(const std::vector<uint8_t> &graph, int H, int W, int sx, int sy, int ex, int ey,
                     GridCost *cost, std::vector<int> *path);
 
-// The additive disjoint partitions for an n x n board (n = 3 or 4).
+// The additive disjoint partitions for an n x n board (n = 3, 4 or 5).
 std::vector<std::vector<int>> pdb_partitions(int n);
 size_t pdb_table_size(int n, size_t k);
 // uAstar's cache file name for one partition: database_<n>_<t1>_<t2>...bin

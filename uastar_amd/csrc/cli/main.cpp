@@ -19,7 +19,7 @@ static const char *kUsage =
     "uastar -- A* search on an MI355X (GA*: many per-wavefront heaps searched in rounds).\n"
     "Two problems:\n"
     "  --pathway   shortest 8-direction path on a grid (orthogonal step 1, diagonal sqrt 2)\n"
-    "  --puzzle    sliding-tile puzzle (3x3 or 4x4) with an additive pattern database\n"
+    "  --puzzle    sliding-tile puzzle (3x3, 4x4 or 5x5) with an additive pattern database\n"
     "\n"
     "Options:\n"
     "  -h, --help                 this message\n"

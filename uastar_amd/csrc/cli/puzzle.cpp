@@ -1,5 +1,5 @@
 // puzzle.cpp -- the command line's sliding-tile puzzle: uAstar's Puzzle (puzzle.cuh,
-// puzzle.cu) for n = 3 and 4 on libuastar, with the pattern database shared between
+// puzzle.cu) for n = 3, 4 and 5 on libuastar, with the pattern database shared between
 // the CPU and the GPU side through uAstar's database_*.bin cache files.
 #include <cstdio>
 #include <iostream>
@@ -179,8 +179,8 @@ std::unique_ptr<Problem> make_puzzle(const Options &o) {
         return nullptr;
     }
     int n = o.width > 0 ? o.width : o.height;
-    if (n != 3 && n != 4) {
-        std::printf("This build solves the 3x3 and 4x4 puzzles.\n");
+    if (n < 3 || n > 5) {
+        std::printf("Currently we can only solve N=3~5 puzzle problem\n");
         return nullptr;
     }
     return std::make_unique<Puzzle>(o, n);

@@ -102,11 +102,19 @@ struct QueryDesc {
     uint64_t *best;          // grid: per cell (gkey << 4 | dir);  puzzle: hash keys
     uint32_t *vals;          // puzzle: hash values (g << 4 | dir)
     uint64_t *heap;          // heap node storage, heaps * (heap_cap-1) * 64 u64
-    uint64_t start_state;    // puzzle
+    uint64_t start_state;    // puzzle board (n = 5: its first word)
+    uint64_t start_state2;   // n = 5: the board's second word
+    const uint8_t *pdb;      // puzzle: the pattern databases of side n
     uint64_t *resume;        // serving: resume record (resume_words(heaps) u64), or null
     int32_t round_budget;    // serving: at most this many rounds per launch (0 = no limit)
     int32_t cls;             // host side: instance class of the launch (CLS_*)
 };
+
+// Problem sets of the search kernel: SET_MAIN solves grids and 3x3 / 4x4 puzzles,
+// SET_FIVE 5x5 puzzles. Each set is its own kernel, so that the 5x5 expansion does not
+// raise the register demand of the others (one kernel holding both spills heavily).
+constexpr int SET_MAIN = 0, SET_FIVE = 1;
+UA_HD inline int set_of(const QueryDesc &q) { return q.kind == KIND_PUZZLE && q.n == 5 ? SET_FIVE : SET_MAIN; }
 
 // The search kernel runs blocks of 16 wavefronts (1024 threads). A team is a group of
 // whole blocks: 16, 32, 64 ... waves (= heaps).

@@ -63,9 +63,9 @@ struct Tuning {
     int grid_large_waves = 16;    // ... and at or above grid_large_cells
     int64_t grid_large_cells = 1ll << 22;
     int puzzle_waves = 64;        // waves (= heaps) per batched / served puzzle query
-    int puzzle_hash_log2[5] = {0, 0, 0, 17, 24};   // batched / served puzzles, by n
+    int puzzle_hash_log2[6] = {0, 0, 0, 17, 24, 22};   // batched / served puzzles, by n
     int plain_heaps[2] = {0, 0};  // plain whole-GPU solve, by kind (grid, puzzle); 0 = 4 per CU
-    int plain_hash_log2[5] = {0, 0, 0, 19, 24};    // plain puzzles, by n
+    int plain_hash_log2[6] = {0, 0, 0, 19, 24, 24};    // plain puzzles, by n
     bool tuned = false;
 };
 }  // namespace uart
@@ -79,8 +79,7 @@ struct StepGraph {
     hipGraphExec_t x = nullptr;
     hipGraphNode_t h2d = nullptr, kern = nullptr, d2h = nullptr;
     void *io = nullptr, *host = nullptr, *segs = nullptr;
-    const void *pdb3 = nullptr, *pdb4 = nullptr;
-    int mode = -1, blocks = 0;
+    int mode = -1, set = 0, blocks = 0;
     void release();
 };
 }  // namespace uart
@@ -95,14 +94,14 @@ struct ua_ctx {
     std::string err;
     int num_cu = 0;
     size_t total_mem = 0;
-    uart::DevBuf pdb[5];
+    uart::DevBuf pdb[6];  // by side n (3, 4, 5)
     uart::DevBuf d_added;
     // launches of k_teams (step.cpp): the I/O block, the groups' insert buffers, the
     // pinned staging copy of the I/O block and one graph per mode
     uart::DevBuf io, segs;
     char *h_io = nullptr;
     size_t h_io_n = 0;
-    uart::StepGraph graph[8];        // by (mode, block count); reused round robin
+    uart::StepGraph graph[8];        // by (mode, problem set, block count); reused round robin
     int graph_next = 0;
     bool graphs = false;             // replay launches from graphs (UA_FLAG_GRAPH)
     std::string graph_err;           // why graphs were switched off, if they were
@@ -151,6 +150,7 @@ int grid_fill_desc(ua_ctx *c, const ua_grid *g, int sx, int sy, int ex, int ey, 
 bool board_ok(int n, const uint8_t *t);
 bool board_solvable(int n, const uint8_t *t);
 uint64_t pack_board(int n, const uint8_t *t);
+void pack_board5(const uint8_t *t, uint64_t *w0, uint64_t *w1);
 int puzzle_fill_desc(ua_ctx *c, int n, const uint8_t *tiles, int heaps, int hash_log2, QueryDesc &q, Plan &p);
 int ensure_pdb(ua_ctx *c, int n);
 void fill_result(const QueryDesc &q, const QueryResult &r, ua_result *o);
@@ -161,10 +161,10 @@ int class_of(ua_ctx *c, int kind, int64_t cells);
 int plain_heaps(ua_ctx *c, int kind);
 int batch_hash_log2(ua_ctx *c, int n);
 int plain_hash_log2(ua_ctx *c, int n);
-// One k_teams launch of `blocks` blocks (step.cpp): the shape, the nq descriptors and
+// One k_teams launch of `blocks` blocks for problem set `set` (step.cpp): the shape, the nq descriptors and
 // the results reported for queries the launch does not reach (init, or zeros) go to the
 // device in one copy; the results come back into out. Sets c->last_kernel_ms.
-int run_step(ua_ctx *c, int mode, int blocks, const LaunchShape &shape, const QueryDesc *descs,
+int run_step(ua_ctx *c, int mode, int set, int blocks, const LaunchShape &shape, const QueryDesc *descs,
              const QueryResult *init, int nq, QueryResult *out);
 void release_steps(ua_ctx *c);
 // mode: 1 = a ua_batch_solve sub-batch, 2 = a serving step (names the kernel in profiles).

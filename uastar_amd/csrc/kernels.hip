@@ -14,16 +14,17 @@ namespace ua {
 // from a counter. Plain solve: one group of every block and one query. Batch / serve:
 // one-block teams for grids (workgroup scope), multi-block groups for 15-puzzles. MODE
 // only names the caller (0 plain, 1 batch, 2 serving step), so that profiles report
-// the three uses as separate kernels. The shape, the counters, the TeamVars and the
+// the three uses as separate kernels; SET is the problem set (SET_MAIN: grids and 3x3 /
+// 4x4 puzzles, SET_FIVE: 5x5 puzzles). The shape, the counters, the TeamVars and the
 // queries come from the launch's I/O block (StepHeader), so the arguments of a launch
 // stay the same from step to step; blocks past the shape's own return at once.
-template <int MODE>
-__global__ __launch_bounds__(1024) void k_teams(char *io, uint64_t *segs, const uint8_t *pdb3,
-                                                const uint8_t *pdb4) {
+template <int MODE, int SET>
+__global__ __launch_bounds__(1024) void k_teams(char *io, uint64_t *segs) {
     StepHeader *h = reinterpret_cast<StepHeader *>(io);
     if ((int)blockIdx.x >= shape_blocks(h->shape)) return;
-    group_block(reinterpret_cast<const QueryDesc *>(io + h->qd_off), reinterpret_cast<QueryResult *>(io + h->qr_off),
-                h->shape, h->qhead, reinterpret_cast<TeamVars *>(io + h->tv_off), segs, pdb3, pdb4);
+    group_block<SET>(reinterpret_cast<const QueryDesc *>(io + h->qd_off),
+                     reinterpret_cast<QueryResult *>(io + h->qr_off), h->shape, h->qhead,
+                     reinterpret_cast<TeamVars *>(io + h->tv_off), segs);
 }
 
 // ------------------------------------------------------------------ path walks
@@ -76,6 +77,38 @@ __global__ void k_puzzle_moves(const uint64_t *keys, const uint32_t *vals, int h
         int i = 0;
         for (; i < PuzzleProb::MAXPROBE && keys[slot] != b; ++i) slot = (slot + 1) & hmask;
         if (i == PuzzleProb::MAXPROBE) break;
+    }
+    *len = -1;
+}
+
+// The same walk over two-word 5x5 boards.
+__global__ void k_puzzle_moves5(const uint64_t *keys, const uint32_t *vals, int hash_log2, uint64_t goal_slot,
+                                int32_t *dirs, int64_t cap, int64_t *len) {
+    if (threadIdx.x || blockIdx.x) return;
+    uint64_t hmask = (1ull << hash_log2) - 1;
+    uint64_t slot = goal_slot;
+    uint64_t b0 = keys[2 * slot], b1 = keys[2 * slot + 1];
+    int64_t c = 0;
+    while (c <= cap) {
+        int d = (int)(vals[slot] & 15u);
+        if (d == 15) {
+            *len = c;
+            return;
+        }
+        if (d > 3 || c == cap) break;
+        dirs[c++] = d;
+        int bl = pos5(b0, b1, 0);
+        int px = bl / 5 - kDX[d], py = bl % 5 - kDY[d];
+        if (px < 0 || px >= 5 || py < 0 || py >= 5) break;
+        int pb = px * 5 + py;
+        uint64_t t = cell5(b0, b1, pb);
+        xor5(b0, b1, pb, t);
+        xor5(b0, b1, bl, t);
+        slot = mix64(b0 ^ mix64(b1)) & hmask;
+        int i = 0;
+        for (; i < PuzzleProb5::MAXPROBE && (keys[2 * slot] != b0 || keys[2 * slot + 1] != b1); ++i)
+            slot = (slot + 1) & hmask;
+        if (i == PuzzleProb5::MAXPROBE) break;
     }
     *len = -1;
 }
@@ -236,17 +269,20 @@ hipError_t launch_grid_pack(const uint8_t *bytes, uint64_t *bm, int H, int W, in
 }
 
 hipError_t teams_max_blocks(int *per_cu) {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_teams<0>, kBlockWaves * 64, 0);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_teams<0, SET_MAIN>, kBlockWaves * 64, 0);
 }
 
-const void *teams_fn(int mode) {
-    return mode == 2 ? (const void *)k_teams<2> : mode == 1 ? (const void *)k_teams<1> : (const void *)k_teams<0>;
+const void *teams_fn(int mode, int set) {
+    if (set == SET_FIVE)
+        return mode == 2 ? (const void *)k_teams<2, SET_FIVE> : mode == 1 ? (const void *)k_teams<1, SET_FIVE>
+                                                                          : (const void *)k_teams<0, SET_FIVE>;
+    return mode == 2 ? (const void *)k_teams<2, SET_MAIN> : mode == 1 ? (const void *)k_teams<1, SET_MAIN>
+                                                                      : (const void *)k_teams<0, SET_MAIN>;
 }
 
-hipError_t launch_teams(char *io, int blocks, uint64_t *segs, const uint8_t *pdb3, const uint8_t *pdb4, int mode,
-                        hipStream_t st) {
-    void *args[] = {(void *)&io, (void *)&segs, (void *)&pdb3, (void *)&pdb4};
-    return hipLaunchCooperativeKernel(teams_fn(mode), dim3(blocks), dim3(kBlockWaves * 64), args, 0, st);
+hipError_t launch_teams(char *io, int blocks, uint64_t *segs, int mode, int set, hipStream_t st) {
+    void *args[] = {(void *)&io, (void *)&segs};
+    return hipLaunchCooperativeKernel(teams_fn(mode, set), dim3(blocks), dim3(kBlockWaves * 64), args, 0, st);
 }
 
 hipError_t launch_grid_path(const uint64_t *best, int H, int W, int sx, int sy, int ex, int ey,
@@ -258,7 +294,10 @@ hipError_t launch_grid_path(const uint64_t *best, int H, int W, int sx, int sy, 
 hipError_t launch_puzzle_moves(int n, const uint64_t *keys, const uint32_t *vals, int hash_log2,
                                uint64_t goal_slot, int32_t *dirs, int64_t cap, int64_t *len,
                                hipStream_t st) {
-    if (n == 4)
+    if (n == 5)
+        hipLaunchKernelGGL(k_puzzle_moves5, dim3(1), dim3(64), 0, st, keys, vals, hash_log2, goal_slot, dirs, cap,
+                           len);
+    else if (n == 4)
         hipLaunchKernelGGL(k_puzzle_moves<4>, dim3(1), dim3(64), 0, st, keys, vals, hash_log2, goal_slot,
                            dirs, cap, len);
     else
@@ -289,6 +328,14 @@ static hipError_t pdb_one(uint8_t *tab, uint64_t size, unsigned long long *d_add
 }
 
 hipError_t build_pdb(int n, uint8_t *pdb, unsigned long long *d_added, hipStream_t st) {
+    if (n == 5) {
+        const uint64_t sz = 127512000ull;  // 25!/19!
+        hipError_t e = pdb_one<5, 0>(pdb + PSpec<5>::OFF[0], sz, d_added, st);
+        if (!e) e = pdb_one<5, 1>(pdb + PSpec<5>::OFF[1], sz, d_added, st);
+        if (!e) e = pdb_one<5, 2>(pdb + PSpec<5>::OFF[2], sz, d_added, st);
+        if (!e) e = pdb_one<5, 3>(pdb + PSpec<5>::OFF[3], sz, d_added, st);
+        return e;
+    }
     if (n == 4) {
         hipError_t e = pdb_one<4, 0>(pdb, PSpec<4>::OFF[1], d_added, st);
         if (e) return e;

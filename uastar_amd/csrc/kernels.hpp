@@ -10,12 +10,12 @@ hipError_t launch_grid_gen(uint64_t *bm, int H, int W, int wpr, uint64_t seed, u
 hipError_t launch_grid_pack(const uint8_t *bytes, uint64_t *bm, int H, int W, int wpr, hipStream_t st);
 // The search kernel (k_teams): `blocks` blocks of kBlockWaves wavefronts, cooperative,
 // over the I/O block io (StepHeader, then TeamVars, descriptors and results). mode
-// names the caller in profiles: 0 plain solve, 1 batch, 2 serving step. teams_fn is the
-// kernel of a mode, for a graph's kernel node (arguments io, segs, pdb3, pdb4).
+// names the caller in profiles: 0 plain solve, 1 batch, 2 serving step; set is the
+// problem set (0: grids and 3x3 / 4x4 puzzles, 1: 5x5 puzzles). teams_fn is the kernel
+// of a mode and set, for a graph's kernel node (arguments io, segs).
 hipError_t teams_max_blocks(int *per_cu);
-const void *teams_fn(int mode);
-hipError_t launch_teams(char *io, int blocks, uint64_t *segs, const uint8_t *pdb3, const uint8_t *pdb4, int mode,
-                        hipStream_t st);
+const void *teams_fn(int mode, int set);
+hipError_t launch_teams(char *io, int blocks, uint64_t *segs, int mode, int set, hipStream_t st);
 hipError_t launch_grid_path(const uint64_t *best, int H, int W, int sx, int sy, int ex, int ey,
                             int32_t *ids, int64_t cap, int64_t *len, hipStream_t st);
 hipError_t launch_puzzle_moves(int n, const uint64_t *keys, const uint32_t *vals, int hash_log2,

@@ -75,7 +75,7 @@ int grid_fill_desc(ua_ctx *c, const ua_grid *g, int sx, int sy, int ex, int ey, 
 }
 
 bool board_ok(int n, const uint8_t *t) {
-    if (n != 3 && n != 4) return false;
+    if (n < 3 || n > 5) return false;
     std::vector<int> seen(n * n, 0);
     for (int i = 0; i < n * n; ++i) {
         if (t[i] >= n * n || seen[t[i]]) return false;
@@ -111,11 +111,25 @@ uint64_t pack_board(int n, const uint8_t *t) {
     return s;
 }
 
+// 5x5 boards as PuzzleStorage<5> (puzzle/storage.hpp:144-170): cells 0..11 in w0, cells
+// 12..23 in w1, five bits each; cell 24's low four bits at w0 bit 60, its bit 4 at w1 bit 60
+void pack_board5(const uint8_t *t, uint64_t *w0, uint64_t *w1) {
+    uint64_t a = 0, b = 0;
+    for (int i = 0; i < 12; ++i) a |= (uint64_t)t[i] << (5 * i);
+    for (int i = 12; i < 24; ++i) b |= (uint64_t)t[i] << (5 * (i - 12));
+    a |= (uint64_t)(t[24] & 15) << 60;
+    b |= (uint64_t)(t[24] & 16) << 56;
+    *w0 = a;
+    *w1 = b;
+}
+
 int puzzle_fill_desc(ua_ctx *c, int n, const uint8_t *tiles, int heaps, int hash_log2, QueryDesc &q, Plan &p) {
     std::memset(&q, 0, sizeof q);
     q.kind = KIND_PUZZLE;
     q.n = n;
-    q.start_state = pack_board(n, tiles);
+    if (n == 5) pack_board5(tiles, &q.start_state, &q.start_state2);
+    else q.start_state = pack_board(n, tiles);
+    q.pdb = c->pdb[n].as<uint8_t>();
     q.hash_log2 = hash_log2;
     q.heaps = heaps;
     q.cls = CLS_PUZZLE;
@@ -124,16 +138,17 @@ int puzzle_fill_desc(ua_ctx *c, int n, const uint8_t *tiles, int heaps, int hash
     int64_t per_heap = (items + 64ll * heaps - 1) / (64ll * heaps);
     q.heap_cap = (int)std::max<int64_t>(per_heap + 1, 4);
     p = Plan{};
-    p.best_bytes = (size_t)8 << hash_log2;
+    p.best_bytes = (size_t)(n == 5 ? 16 : 8) << hash_log2;  // keys: one word a board, two for n = 5
     p.vals_bytes = (size_t)4 << hash_log2;
     p.heap_bytes = (size_t)heaps * (size_t)(q.heap_cap - 1) * 512;
     return UA_OK;
 }
 
-static uint64_t pdb_bytes(int n) { return n == 4 ? 576576000ull : 362880ull; }
+// the partitions' tables of side n, concatenated: 9!, 16!/8! + 16!/9!, 4 x 25!/19!
+static uint64_t pdb_bytes(int n) { return n == 5 ? 510048000ull : n == 4 ? 576576000ull : 362880ull; }
 
 int ensure_pdb(ua_ctx *c, int n) {
-    if (n != 3 && n != 4) return fail(c, UA_ERR_INVALID, "puzzle side must be 3 or 4");
+    if (n < 3 || n > 5) return fail(c, UA_ERR_INVALID, "puzzle side must be 3, 4 or 5");
     if (c->pdb[n].p) return UA_OK;
     HIPCHK(c, c->pdb[n].ensure(pdb_bytes(n)));
     HIPCHK(c, c->d_added.ensure(sizeof(unsigned long long)));
@@ -187,10 +202,10 @@ int plain_heaps(ua_ctx *c, int kind) {
 
 // Hash slots (log2) per puzzle query: batched / served, and plain.
 int batch_hash_log2(ua_ctx *c, int n) {
-    return c->cfg.hash_log2 > 0 ? c->cfg.hash_log2 : c->tune.puzzle_hash_log2[n == 3 ? 3 : 4];
+    return c->cfg.hash_log2 > 0 ? c->cfg.hash_log2 : c->tune.puzzle_hash_log2[n >= 3 && n <= 5 ? n : 4];
 }
 int plain_hash_log2(ua_ctx *c, int n) {
-    return c->cfg.hash_log2 > 0 ? c->cfg.hash_log2 : c->tune.plain_hash_log2[n == 3 ? 3 : 4];
+    return c->cfg.hash_log2 > 0 ? c->cfg.hash_log2 : c->tune.plain_hash_log2[n >= 3 && n <= 5 ? n : 4];
 }
 
 static int run_wide(ua_ctx *c, QueryDesc &q, const Plan &p) {
@@ -216,7 +231,7 @@ static int run_wide(ua_ctx *c, QueryDesc &q, const Plan &p) {
     shape.segcap[0] = segcap;
     for (int k = 1; k < kClasses; ++k) shape.bpg[k] = 1;
     QueryResult r{};
-    int rc = run_step(c, 0, blocks, shape, &q, nullptr, 1, &r);
+    int rc = run_step(c, 0, set_of(q), blocks, shape, &q, nullptr, 1, &r);
     if (rc) return rc;
     c->last_q = q;
     c->last_r = r;
@@ -388,7 +403,7 @@ extern "C" int ua_puzzle_build_pdb(ua_ctx *c, int n) {
 
 extern "C" int ua_puzzle_load_pdb(ua_ctx *c, int n, const uint8_t *host, uint64_t size) {
     if (!c || !host) return fail(c, UA_ERR_INVALID, "null argument");
-    if (n != 3 && n != 4) return fail(c, UA_ERR_INVALID, "puzzle side must be 3 or 4");
+    if (n < 3 || n > 5) return fail(c, UA_ERR_INVALID, "puzzle side must be 3, 4 or 5");
     if (size != pdb_bytes(n)) return fail(c, UA_ERR_INVALID, "a %dx%d PDB holds %llu bytes", n, n,
                                           (unsigned long long)pdb_bytes(n));
     HIPCHK(c, hipStreamSynchronize(c->st));
@@ -484,12 +499,12 @@ int team_waves(ua_ctx *c, int kind, int64_t cells) {
     return blocks * kBlockWaves;
 }
 
-// Runs the planned queries in one cooperative launch of k_teams. Each query's class
-// (CLS_*) gets groups of heaps / kBlockWaves blocks; the resident blocks are shared
-// among the classes in proportion to their demand (queries x blocks per team).
-// Results come back in the order of descs.
-int run_batch(ua_ctx *c, std::vector<QueryDesc> &descs, std::vector<QueryResult> &res, int mode,
-              int64_t budget_ticks, const std::vector<QueryResult> *init) {
+// Runs the planned queries of one problem set in one cooperative launch of k_teams.
+// Each query's class (CLS_*) gets groups of heaps / kBlockWaves blocks; the resident
+// blocks are shared among the classes in proportion to their demand (queries x blocks
+// per team). Results come back in the order of descs.
+static int run_set(ua_ctx *c, int set, std::vector<QueryDesc> &descs, std::vector<QueryResult> &res, int mode,
+                   int64_t budget_ticks, const std::vector<QueryResult> *init) {
     const int nq = (int)descs.size();
     res.assign(nq, QueryResult{});
     if (!nq) return UA_OK;
@@ -550,9 +565,35 @@ int run_batch(ua_ctx *c, std::vector<QueryDesc> &descs, std::vector<QueryResult>
     }
     // every resident block: the grid of a launch stays the same from step to step
     std::vector<QueryResult> sres(nq);
-    int rc = run_step(c, mode, resident, shape, sorted.data(), init ? sinit.data() : nullptr, nq, sres.data());
+    int rc = run_step(c, mode, set, resident, shape, sorted.data(), init ? sinit.data() : nullptr, nq, sres.data());
     if (rc) return rc;
     for (int k = 0; k < nq; ++k) res[order[k]] = sres[k];
+    return UA_OK;
+}
+
+// The planned queries in one launch per problem set present (5x5 puzzles run in a
+// launch of their own after the rest); last_kernel_ms is the launches' sum.
+int run_batch(ua_ctx *c, std::vector<QueryDesc> &descs, std::vector<QueryResult> &res, int mode,
+              int64_t budget_ticks, const std::vector<QueryResult> *init) {
+    std::vector<int> idx[2];
+    for (int i = 0; i < (int)descs.size(); ++i) idx[set_of(descs[i])].push_back(i);
+    if (idx[SET_FIVE].empty()) return run_set(c, SET_MAIN, descs, res, mode, budget_ticks, init);
+    if (idx[SET_MAIN].empty()) return run_set(c, SET_FIVE, descs, res, mode, budget_ticks, init);
+    res.assign(descs.size(), QueryResult{});
+    double kms = 0;
+    for (int set : {SET_MAIN, SET_FIVE}) {
+        std::vector<QueryDesc> d;
+        std::vector<QueryResult> r, in;
+        for (int i : idx[set]) {
+            d.push_back(descs[i]);
+            if (init) in.push_back((*init)[i]);
+        }
+        int rc = run_set(c, set, d, r, mode, budget_ticks, init ? &in : nullptr);
+        if (rc) return rc;
+        kms += c->last_kernel_ms;
+        for (size_t k = 0; k < idx[set].size(); ++k) res[idx[set][k]] = r[k];
+    }
+    c->last_kernel_ms = kms;
     return UA_OK;
 }
 

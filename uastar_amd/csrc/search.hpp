@@ -304,6 +304,21 @@ struct PSpec<4> {
     static constexpr uint64_t OFF[2] = {0, 518918400ull};
 };
 
+template <>
+struct PSpec<5> {
+    static constexpr int NP = 4;
+    static constexpr int K[4] = {6, 6, 6, 6};
+    static constexpr int T[4][8] = {{3, 4, 5, 10, 15, 20, 0, 0},
+                                    {2, 1, 6, 11, 16, 21, 0, 0},
+                                    {7, 8, 9, 12, 17, 22, 0, 0},
+                                    {13, 14, 18, 19, 23, 24, 0, 0}};
+    static constexpr uint32_t M[4][8] = {{5100480, 212520, 9240, 420, 20, 1, 0, 0},
+                                         {5100480, 212520, 9240, 420, 20, 1, 0, 0},
+                                         {5100480, 212520, 9240, 420, 20, 1, 0, 0},
+                                         {5100480, 212520, 9240, 420, 20, 1, 0, 0}};
+    static constexpr uint64_t OFF[4] = {0, 127512000ull, 255024000ull, 382536000ull};
+};
+
 __device__ __forceinline__ int nib_pos(uint64_t s, int t) {
     uint64_t x = s ^ (0x1111111111111111ull * (uint64_t)t);
     uint64_t z = (x - 0x1111111111111111ull) & ~x & 0x8888888888888888ull;
@@ -465,6 +480,170 @@ struct PuzzleProb {
             if ((old >> 4) <= ng) continue;
             ++n_ins;
             if (ns == goal_b) {
+                uint64_t gk = ((uint64_t)ng << 32) | ns_slot;
+                goalmin = goalmin < gk ? goalmin : gk;
+            } else {
+                s[d] = ((uint64_t)(ng + h) << 56) | ((uint64_t)(255u - ng) << 48) | ns_slot;
+                ++cnt;
+            }
+        }
+    }
+};
+
+// ------------------------------------------------------------------ 5x5 puzzle problem
+// Board of two words as PuzzleStorage<5> (puzzle/storage.hpp:141-217): cells 0..11 at
+// w0 bits 5c..5c+4, cells 12..23 at w1 bits 5(c-12).., cell 24 split into its low four
+// bits at w0 bits 60..63 and its bit 4 at w1 bit 60. Neither word of a board is zero
+// (each holds at least eleven nonzero tiles), so 0 marks an empty hash word.
+constexpr uint64_t kOnes12 = 0x84210842108421ull;   // bit 5i, i < 12
+constexpr uint64_t kLow60 = (1ull << 60) - 1;
+
+__device__ __forceinline__ uint64_t cell5(uint64_t w0, uint64_t w1, int c) {
+    if (c < 12) return (w0 >> (5 * c)) & 31;
+    if (c < 24) return (w1 >> (5 * (c - 12))) & 31;
+    return (w0 >> 60) | ((w1 >> 56) & 16);
+}
+// toggles tile t in cell c (sets it in an empty cell, clears it where it is)
+__device__ __forceinline__ void xor5(uint64_t &w0, uint64_t &w1, int c, uint64_t t) {
+    if (c < 12) w0 ^= t << (5 * c);
+    else if (c < 24) w1 ^= t << (5 * (c - 12));
+    else {
+        w0 ^= (t & 15) << 60;
+        w1 ^= (t & 16) << 56;
+    }
+}
+// cell of tile t: the lowest zero field of (cells ^ t) in either word, else cell 24
+__device__ __forceinline__ int pos5(uint64_t w0, uint64_t w1, int t) {
+    const uint64_t pat = kOnes12 * (uint64_t)t;
+    const uint64_t x0 = (w0 & kLow60) ^ pat, x1 = (w1 & kLow60) ^ pat;
+    const uint64_t z0 = (x0 - kOnes12) & ~x0 & (kOnes12 << 4);
+    const uint64_t z1 = (x1 - kOnes12) & ~x1 & (kOnes12 << 4);
+    if (z0) return __builtin_ctzll(z0) / 5;
+    if (z1) return 12 + __builtin_ctzll(z1) / 5;
+    return 24;
+}
+__device__ __forceinline__ uint32_t rank5(uint64_t w0, uint64_t w1, int p) {
+    using S = PSpec<5>;
+    uint32_t mask = 0, code = 0;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+        int pos = pos5(w0, w1, S::T[p][j]);
+        code += ((uint32_t)pos - __builtin_popcount(mask & ((1u << pos) - 1u))) * S::M[p][j];
+        mask |= 1u << pos;
+    }
+    return code;
+}
+// partition of each tile (tile 0, the blank, in none)
+__constant__ static const int8_t kPart5[25] = {-1, 1, 1, 0, 0, 0, 1, 2, 2, 2, 0, 1, 2,
+                                               3, 3, 0, 1, 2, 3, 3, 0, 1, 2, 3, 3};
+
+// Item key and hash values as PuzzleProb; hash keys are two words per slot
+// (keys[2 slot], keys[2 slot + 1]). A slot is claimed by a CAS of its first word, and
+// the winner stores the second word at once; a prober that finds its first word in a
+// slot whose second word is not yet stored reads it again (a bounded wait). The child's
+// h is the parent's (f - g of its item) with the moved tile's partition term swapped.
+struct PuzzleProb5 {
+    static constexpr int MAXS = 4;
+    static constexpr int MAXPROBE = 256;
+    static constexpr int SPIN = 1 << 16;
+    QueryDesc q;
+    uint64_t hmask;
+    uint64_t g0, g1;
+    __device__ explicit PuzzleProb5(const QueryDesc &qd) : q(qd) {
+        hmask = (1ull << q.hash_log2) - 1;
+        g0 = 0x62d4941cc520c41ull;  // tiles 1..12 on cells 0..11
+        g1 = 0xc5ed5a4e5183dcdull;  // tiles 13..24 on cells 12..23, blank on cell 24
+    }
+    __device__ bool trivial() const { return q.start_state == g0 && q.start_state2 == g1; }
+    __device__ __forceinline__ int h_full(uint64_t w0, uint64_t w1) const {
+        int h = 0;
+#pragma unroll
+        for (int p = 0; p < 4; ++p) h += q.pdb[PSpec<5>::OFF[p] + rank5(w0, w1, p)];
+        return h;
+    }
+    template <class SC>
+    __device__ __forceinline__ uint64_t find_or_insert(const SC &sc, uint64_t b0, uint64_t b1, uint32_t &err) const {
+        uint64_t slot = mix64(b0 ^ mix64(b1)) & hmask;
+        for (int i = 0; i < MAXPROBE; ++i) {
+            uint64_t k0 = sc.load(&q.best[2 * slot]);
+            if (k0 == 0) {
+                uint64_t exp = 0;
+                if (sc.cas(&q.best[2 * slot], exp, b0)) {
+                    sc.store(&q.best[2 * slot + 1], b1);
+                    return slot;
+                }
+                k0 = exp;
+            }
+            if (k0 == b0) {
+                uint64_t k1 = sc.load(&q.best[2 * slot + 1]);
+                for (int w = 0; k1 == 0 && w < SPIN; ++w) k1 = sc.load(&q.best[2 * slot + 1]);
+                if (k1 == 0) {
+                    err |= FLAG_ABORT;
+                    return SENT;
+                }
+                if (k1 == b1) return slot;
+            }
+            slot = (slot + 1) & hmask;
+        }
+        err |= FLAG_CAPACITY;
+        return SENT;
+    }
+    __device__ uint64_t seed(const Scope &sc) const {
+        uint32_t err = 0;
+        uint64_t slot = find_or_insert(sc, q.start_state, q.start_state2, err);
+        if (slot == SENT) return SENT;
+        sc.store(&q.vals[slot], (uint32_t)15);
+        uint64_t h = (uint64_t)h_full(q.start_state, q.start_state2);
+        return (h << 56) | (255ull << 48) | slot;
+    }
+    __device__ bool goal_done(uint64_t goal, uint64_t lb) const {
+        return goal != SENT && (goal >> 32) <= (lb >> 56);
+    }
+    template <class SC>
+    __device__ __forceinline__ void expand(const SC &sc, uint64_t it, uint64_t (&s)[MAXS], int &cnt,
+                                           uint64_t &goalmin, uint32_t &err, int64_t &n_exp, int64_t &n_gen,
+                                           int64_t &n_ins) const {
+        uint64_t slot = it & ((1ull << 48) - 1);
+        if (slot > hmask) {  // never produced by a correct round: fail the query, touch nothing
+            err |= FLAG_ABORT;
+            return;
+        }
+        uint32_t gi = 255u - (uint32_t)((it >> 48) & 255u);
+        int hp = (int)(it >> 56) - (int)gi;
+        uint64_t w0 = sc.load(&q.best[2 * slot]);
+        uint64_t w1 = sc.load(&q.best[2 * slot + 1]);
+        uint32_t v = sc.load(&q.vals[slot]);
+        uint32_t gb = v >> 4;
+        if (gb != gi) return;  // stale
+        ++n_exp;
+        int dprev = (int)(v & 15u);
+        int bl = pos5(w0, w1, 0);
+        int bx = bl / 5, by = bl - bx * 5;
+#pragma unroll
+        for (int d = 0; d < 4; ++d) {
+            if (dprev < 4 && d == (dprev ^ 1)) continue;  // the move back to the parent
+            int nx = bx + kDX[d], ny = by + kDY[d];
+            if (nx < 0 || nx >= 5 || ny < 0 || ny >= 5) continue;
+            ++n_gen;
+            int nb = nx * 5 + ny;
+            uint64_t t = cell5(w0, w1, nb);
+            uint64_t n0 = w0, n1 = w1;
+            xor5(n0, n1, bl, t);
+            xor5(n0, n1, nb, t);
+            const int p = kPart5[t];
+            const uint64_t off = (uint64_t)p * 127512000ull;  // PSpec<5>::OFF[p]
+            int h = hp - (int)q.pdb[off + rank5(w0, w1, p)] + (int)q.pdb[off + rank5(n0, n1, p)];
+            uint32_t ng = gb + 1;
+            if (ng + h > 255u) {
+                err |= FLAG_RANGE;
+                continue;
+            }
+            uint64_t ns_slot = find_or_insert(sc, n0, n1, err);
+            if (ns_slot == SENT) continue;
+            uint32_t old = sc.fmin(&q.vals[ns_slot], (ng << 4) | (uint32_t)d);
+            if ((old >> 4) <= ng) continue;
+            ++n_ins;
+            if (n0 == g0 && n1 == g1) {
                 uint64_t gk = ((uint64_t)ng << 32) | ns_slot;
                 goalmin = goalmin < gk ? goalmin : gk;
             } else {
@@ -663,14 +842,24 @@ __device__ void team_solve(const QueryDesc &q, QueryResult *res, const Team &tea
     }
 }
 
-template <class Team>
-__device__ void dispatch(const QueryDesc &q, QueryResult *res, const uint8_t *pdb3, const uint8_t *pdb4,
-                         const Team &team) {
-    if (q.kind == KIND_GRID) {
+// A query of the other problem set (set_of) fails with ST_INVALID.
+template <int SET, class Team>
+__device__ void dispatch(const QueryDesc &q, QueryResult *res, const Team &team) {
+    if (set_of(q) != SET) {
+        if (team.leader()) {
+            res->status = ST_INVALID;
+            res->found = 0;
+        }
+        return;
+    }
+    if constexpr (SET == SET_FIVE) {
+        PuzzleProb5 P(q);
+        team_solve(q, res, team, P);
+    } else if (q.kind == KIND_GRID) {
         GridProb P(q);
         team_solve(q, res, team, P);
     } else {
-        PuzzleProb P(q, q.n == 3 ? pdb3 : pdb4);
+        PuzzleProb P(q, q.pdb);
         team_solve(q, res, team, P);
     }
 }
@@ -681,9 +870,9 @@ __device__ void dispatch(const QueryDesc &q, QueryResult *res, const uint8_t *pd
 // as one team, until the class's queries are exhausted.
 // With a time budget the group's first hook fixes its deadline; after the deadline it
 // takes no more queries, and the round barrier of the query it runs flags FLAG_TIMEUP.
-template <class Team>
+template <int SET, class Team>
 __device__ void group_loop(const QueryDesc *qs, QueryResult *rs, const Team &team, TeamVars *tv, uint32_t *head,
-                           int nq, int qoff, int64_t budget, const uint8_t *pdb3, const uint8_t *pdb4) {
+                           int nq, int qoff, int64_t budget) {
     for (;;) {
         int64_t t0 = now_ticks();
         if (!team.sync([tv, head, budget] {
@@ -704,7 +893,7 @@ __device__ void group_loop(const QueryDesc *qs, QueryResult *rs, const Team &tea
         if (k >= (uint32_t)nq) break;
         int qi = qoff + k;
         QueryDesc q = qs[qi];
-        dispatch(q, rs + qi, pdb3, pdb4, team);
+        dispatch<SET>(q, rs + qi, team);
         if (team.leader()) {
             rs[qi].t_start = t0;
             rs[qi].t_end = now_ticks();
@@ -717,9 +906,9 @@ constexpr int kLdsSegCap = kBlockWaves * 64 * 8;
 
 // Body of k_teams for one block: find this block's class and group from the launch
 // shape, then run the group's query loop.
+template <int SET = SET_MAIN>
 __device__ __forceinline__ void group_block(const QueryDesc *qs, QueryResult *rs, const LaunchShape &shape,
-                                   uint32_t *qhead, TeamVars *tvs, uint64_t *segs, const uint8_t *pdb3,
-                                   const uint8_t *pdb4) {
+                                            uint32_t *qhead, TeamVars *tvs, uint64_t *segs) {
     int rel = (int)blockIdx.x, cls = 0, group = 0;
     size_t seg_off = 0;
     while (cls + 1 < kClasses && rel >= shape.groups[cls] * shape.bpg[cls]) {
@@ -747,7 +936,7 @@ __device__ __forceinline__ void group_block(const QueryDesc *qs, QueryResult *rs
     m.seg = in_lds ? lds_seg : segs + seg_off;
     m.segcap = segcap;
     GroupTeam team(m, tv, bpg, rel % bpg);
-    group_loop(qs, rs, team, tv, qhead + cls, shape.nq[cls], shape.qoff[cls], shape.budget_ticks, pdb3, pdb4);
+    group_loop<SET>(qs, rs, team, tv, qhead + cls, shape.nq[cls], shape.qoff[cls], shape.budget_ticks);
 }
 
 }  // namespace ua

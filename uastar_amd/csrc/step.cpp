@@ -5,10 +5,10 @@
 // counters, the groups' TeamVars, the query descriptors and the results reported for
 // queries it does not reach -- forms one I/O block (StepHeader first). The host builds
 // the block in pinned memory and it reaches the device in one copy; the kernel's
-// arguments (the block, the insert buffers, the two PDBs) and its grid stay the same
+// arguments (the block and the insert buffers) and its grid stay the same
 // from launch to launch. A launch is five operations on the context's stream: copy in,
 // event, cooperative kernel, event, copy of the results out. With UA_FLAG_GRAPH they are
-// an instantiated hipGraph per (mode, block count), replayed with only the two copy
+// an instantiated hipGraph per (mode, problem set, block count), replayed with only the two copy
 // sizes updated; otherwise they are issued call by call. Where the runtime refuses a
 // part of the graph, the context notes why and issues the calls from then on.
 #include <algorithm>
@@ -54,19 +54,18 @@ Layout layout_of(int groups, int nq) {
 
 // Builds and instantiates the graph of one launch for the current addresses. The copy
 // sizes set here are placeholders: every replay sets its own.
-hipError_t build_graph(ua_ctx *c, StepGraph &sg, int mode, int blocks) {
+hipError_t build_graph(ua_ctx *c, StepGraph &sg, int mode, int set, int blocks) {
     sg.release();
     char *io = c->io.as<char>();
     uint64_t *segs = c->segs.as<uint64_t>();
-    const uint8_t *pdb3 = c->pdb[3].as<uint8_t>(), *pdb4 = c->pdb[4].as<uint8_t>();
     hipError_t e = hipGraphCreate(&sg.g, 0);
     if (e) return e;
     e = hipGraphAddMemcpyNode1D(&sg.h2d, sg.g, nullptr, 0, io, c->h_io, sizeof(StepHeader), hipMemcpyHostToDevice);
     hipGraphNode_t ev0 = nullptr, ev1 = nullptr;
     if (!e) e = hipGraphAddEventRecordNode(&ev0, sg.g, &sg.h2d, 1, c->ev0);
-    void *args[] = {(void *)&io, (void *)&segs, (void *)&pdb3, (void *)&pdb4};
+    void *args[] = {(void *)&io, (void *)&segs};
     hipKernelNodeParams kp{};
-    kp.func = const_cast<void *>(teams_fn(mode));
+    kp.func = const_cast<void *>(teams_fn(mode, set));
     kp.gridDim = dim3(blocks);
     kp.blockDim = dim3(kBlockWaves * 64);
     kp.sharedMemBytes = 0;
@@ -89,9 +88,8 @@ hipError_t build_graph(ua_ctx *c, StepGraph &sg, int mode, int blocks) {
     sg.io = io;
     sg.host = c->h_io;
     sg.segs = segs;
-    sg.pdb3 = pdb3;
-    sg.pdb4 = pdb4;
     sg.mode = mode;
+    sg.set = set;
     sg.blocks = blocks;
     return hipSuccess;
 }
@@ -109,24 +107,24 @@ hipError_t set_copies(ua_ctx *c, StepGraph &sg, const Layout &l) {
 // Replays the launch's graph (building it first if needed). Returns false, with the
 // reason in c->graph_err, if the runtime refuses to build or update it: nothing of the
 // launch has run then.
-bool try_graph(ua_ctx *c, int mode, int blocks, const Layout &l, hipError_t *launch_err) {
+bool try_graph(ua_ctx *c, int mode, int set, int blocks, const Layout &l, hipError_t *launch_err) {
     const int ng = (int)(sizeof c->graph / sizeof c->graph[0]);
     StepGraph *pick = nullptr;
     for (auto &g : c->graph)
-        if (g.x && g.mode == mode && g.blocks == blocks) pick = &g;
+        if (g.x && g.mode == mode && g.set == set && g.blocks == blocks) pick = &g;
     if (!pick) {
         pick = &c->graph[c->graph_next];
         c->graph_next = (c->graph_next + 1) % ng;
     }
     StepGraph &sg = *pick;
-    const bool same = sg.x && sg.mode == mode && sg.io == c->io.p && sg.host == c->h_io && sg.segs == c->segs.p &&
-                      sg.pdb3 == c->pdb[3].p && sg.pdb4 == c->pdb[4].p && sg.blocks == blocks;
+    const bool same = sg.x && sg.mode == mode && sg.set == set && sg.io == c->io.p && sg.host == c->h_io && sg.segs == c->segs.p &&
+                      sg.blocks == blocks;
     hipError_t e = hipSuccess;
-    if (!same) e = build_graph(c, sg, mode, blocks);
+    if (!same) e = build_graph(c, sg, mode, set, blocks);
     if (!e) {
         e = set_copies(c, sg, l);
         if (e && same) {  // an update the instantiated graph does not take: build it afresh once
-            e = build_graph(c, sg, mode, blocks);
+            e = build_graph(c, sg, mode, set, blocks);
             if (!e) e = set_copies(c, sg, l);
         }
     }
@@ -142,11 +140,12 @@ bool try_graph(ua_ctx *c, int mode, int blocks, const Layout &l, hipError_t *lau
 
 }  // namespace
 
-int run_step(ua_ctx *c, int mode, int blocks, const LaunchShape &shape, const QueryDesc *descs,
+int run_step(ua_ctx *c, int mode, int set, int blocks, const LaunchShape &shape, const QueryDesc *descs,
              const QueryResult *init, int nq, QueryResult *out) {
     using Clock = std::chrono::steady_clock;
     const auto t0 = Clock::now();
-    if (mode < 0 || mode > 2) return fail(c, UA_ERR_INVALID, "bad launch mode %d", mode);
+    if (mode < 0 || mode > 2 || (set != SET_MAIN && set != SET_FIVE))
+        return fail(c, UA_ERR_INVALID, "bad launch mode %d / set %d", mode, set);
     int per_cu = 0;
     HIPCHK(c, teams_max_blocks(&per_cu));
     const int resident = std::max(1, c->num_cu * std::max(1, per_cu));
@@ -190,7 +189,7 @@ int run_step(ua_ctx *c, int mode, int blocks, const LaunchShape &shape, const Qu
     bool replayed = false;
     if (c->graphs) {
         hipError_t le = hipSuccess;
-        replayed = try_graph(c, mode, blocks, l, &le);
+        replayed = try_graph(c, mode, set, blocks, l, &le);
         if (!replayed) c->graphs = false;
         else HIPCHK(c, le);
     }
@@ -198,8 +197,7 @@ int run_step(ua_ctx *c, int mode, int blocks, const LaunchShape &shape, const Qu
         char *io = c->io.as<char>();
         HIPCHK(c, hipMemcpyAsync(io, c->h_io, l.end, hipMemcpyHostToDevice, c->st));
         HIPCHK(c, hipEventRecord(c->ev0, c->st));
-        HIPCHK(c, launch_teams(io, blocks, c->segs.as<uint64_t>(), c->pdb[3].as<uint8_t>(), c->pdb[4].as<uint8_t>(),
-                               mode, c->st));
+        HIPCHK(c, launch_teams(io, blocks, c->segs.as<uint64_t>(), mode, set, c->st));
         HIPCHK(c, hipEventRecord(c->ev1, c->st));
         HIPCHK(c, hipMemcpyAsync(c->h_io + l.qr_off, io + l.qr_off, l.end - l.qr_off, hipMemcpyDeviceToHost, c->st));
     }
