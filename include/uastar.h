@@ -66,6 +66,8 @@ typedef struct {
     double kernel_ms;    /* duration of the search launch, from HIP events on the context's stream
                             (a batch: the launch that ran this query, shared by its sub-batch) */
     double latency_ms;   /* serve: completion - submission; batch: completion - launch */
+    int32_t optimal;     /* 1: the cost is proven optimal (every exact search that finds a path) */
+    int32_t weight;      /* weight on h of the search that found the answer (1 = plain A*) */
 } ua_result;
 
 /* Device context: owns every device allocation. Replaces the device set-up of
@@ -109,6 +111,14 @@ int ua_puzzle_load_pdb(ua_ctx *ctx, int n, const uint8_t *host, uint64_t size);
 /* Plain solve of one board (row-major, 0 = blank): replaces GPUPuzzleSolver<N>::solve
  * (puzzle/GPU-solver.cuh:197-274). Unsolvable boards return UA_NO_PATH at once. */
 int ua_puzzle_solve(ua_ctx *ctx, int n, const uint8_t *tiles, ua_result *res);
+/* The memory-bounded scheme the reference describes for puzzles too large for plain A*
+ * (README.md:30-34) but never built. The search's hash and heaps get at most mem_bytes
+ * of device memory. It first runs plain A*; if that does not fit, it runs weighted
+ * searches (keys (g >> k) + h, k = 1, 2, 3: weight 2, 4, 8 on h) on fewer heaps in the
+ * same memory, dropping successors that do not fit, and returns the first solution
+ * found. Then
+ * res->optimal = 0 and res->weight gives the weight. UA_ERR_CAPACITY if none is found. */
+int ua_puzzle_solve_bounded(ua_ctx *ctx, int n, const uint8_t *tiles, uint64_t mem_bytes, ua_result *res);
 /* Blank moves of the last ua_puzzle_solve (0..3 = (+1,0),(-1,0),(0,+1),(0,-1)):
  * replaces GPUPuzzleSolver<N>::getSolution (puzzle/GPU-solver.cuh:276-287). */
 int ua_puzzle_moves(ua_ctx *ctx, int32_t *dirs, int64_t cap, int64_t *len);

@@ -25,7 +25,7 @@ class QueryDesc(C.Structure):
                                           "qshift", "heap_cap", "hash_log2", "max_rounds", "heaps")] + [
         ("bitmap", C.c_void_p), ("best", C.c_void_p), ("vals", C.c_void_p), ("heap", C.c_void_p),
         ("start_state", C.c_uint64), ("start_state2", C.c_uint64), ("pdb", C.c_void_p), ("resume", C.c_void_p),
-        ("round_budget", C.c_int32), ("cls", C.c_int32)]
+        ("round_budget", C.c_int32), ("cls", C.c_int32), ("wshift", C.c_int32), ("bounded", C.c_int32)]
 
 RESUME_WORDS_4 = 8 + 4 * 130
 
@@ -275,3 +275,30 @@ def test_emulated_batch_puzzles5(emu5, oracle):
     res = (QueryResult * 1)()
     assert emu5.emu_batch5(arr, res, 1, 4) == 0
     assert res[0].status == -1
+
+
+def test_emulated_bounded_weighted_puzzles(emu, oracle):
+    """The memory-bounded scheme's device side. Weighted keys (g >> wshift) + h with
+    first-goal termination give solutions no shorter than the optimum, of the same
+    parity. With bounded = 1 a hash too small for the search drops successors instead
+    of failing: the query ends without an error, FLAG_PRUNED set; without bounded it
+    fails with ST_CAPACITY."""
+    pdb3 = oracle.pdb_full(3)
+    b = Batch(waves=4)
+    exp = []
+    for s in range(3):
+        t = oracle.puzzle_random(3, 4400 + s)
+        exp.append(oracle.puzzle_solve(3, t, pdb3, "astar")[0])
+        b.add_puzzle(3, t, pdb3, hash_log2=15)
+        b.descs[-1].wshift = 1 + s
+    for s, (ws, bounded) in enumerate([(1, 1), (0, 1), (0, 0)]):
+        b.add_puzzle(3, oracle.puzzle_random(3, 4400 + s), pdb3, hash_log2=9)
+        b.descs[-1].wshift, b.descs[-1].bounded = ws, bounded
+    res = b.run(emu)
+    for r, e in zip(res[:3], exp):
+        assert r.status == 0 and r.found and r.flags == 0, r.status
+        steps = r.goal >> 32
+        assert steps >= e and (steps - e) % 2 == 0, (steps, e)
+    for r in res[3:5]:  # bounded: pruned, no error
+        assert r.status in (0, 1) and r.flags & 16
+    assert res[5].status == -3  # unbounded: capacity error

@@ -110,3 +110,27 @@ def test_batch_and_serve_mix_5x5(c5, oracle):
     check(res)
     for h, *_ in grids:
         h.free()
+
+
+def test_memory_bounded_scheme(c5):
+    """The memory-bounded scheme (README.md:30-34 of the reference, never built there): a
+    board that plain A* solves in ample memory comes back optimal; in a bound too small
+    for plain A* the weighted, pruning search returns a valid solution, flagged as not
+    proven optimal, no shorter than the optimum and of the same parity."""
+    hard = max(_rows(10_000_000), key=lambda r: r["expanded"])
+    board = np.array(hard["board"], np.uint8)
+    _say(f"bounded 5x5 seed {hard['seed']} in 4 GiB")
+    r = c5.puzzle_solve_bounded(5, board, 4 << 30)
+    assert r["status"] == 0 and r["optimal"] == 1 and r["weight"] == 1 and r["steps"] == hard["steps"]
+    for mem in (256 << 20, 64 << 20):
+        _say(f"bounded 5x5 seed {hard['seed']} in {mem >> 20} MiB")
+        r = c5.puzzle_solve_bounded(5, board, mem)
+        assert r["status"] == 0 and r["found"], r
+        assert r["optimal"] == 0 and r["weight"] >= 2
+        assert r["steps"] >= hard["steps"] and (r["steps"] - hard["steps"]) % 2 == 0
+        mv = c5.puzzle_moves()
+        assert len(mv) == r["steps"] and replay_moves(5, hard["board"], mv) == goal_board(5)
+    # a 15-puzzle too: plain A* fits a generous bound
+    row = [r for r in json.load(open(os.path.join(GOLDEN, "c3_puzzles.json"))) if r["expanded"] < 300_000][0]
+    r = c5.puzzle_solve_bounded(4, np.array(row["board"], np.uint8), 1 << 30)
+    assert r["status"] == 0 and r["optimal"] == 1 and r["steps"] == row["steps"]

@@ -48,7 +48,8 @@ class Result(C.Structure):
     _fields_ = [("status", C.c_int32), ("found", C.c_int32), ("orth", C.c_int64), ("diag", C.c_int64),
                 ("steps", C.c_int64), ("cost", C.c_double), ("rounds", C.c_int64), ("pops", C.c_int64),
                 ("expanded", C.c_int64), ("generated", C.c_int64), ("inserted", C.c_int64),
-                ("device_ms", C.c_double), ("kernel_ms", C.c_double), ("latency_ms", C.c_double)]
+                ("device_ms", C.c_double), ("kernel_ms", C.c_double), ("latency_ms", C.c_double),
+                ("optimal", C.c_int32), ("weight", C.c_int32)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -100,6 +101,7 @@ def lib():
             "ua_puzzle_load_pdb": (C.c_int, [vp, C.c_int, vp, u64]),
             "ua_puzzle_solve": (C.c_int, [vp, C.c_int, vp, C.POINTER(Result)]),
             "ua_puzzle_moves": (C.c_int, [vp, vp, i64, C.POINTER(i64)]),
+            "ua_puzzle_solve_bounded": (C.c_int, [vp, C.c_int, vp, u64, C.POINTER(Result)]),
             "ua_batch_solve": (C.c_int, [vp, C.POINTER(Query), i64, C.POINTER(Result)]),
             "ua_batch_path": (C.c_int, [vp, i64, vp, i64, C.POINTER(i64)]),
             "ua_serve_submit": (C.c_int, [vp, C.POINTER(Query), C.POINTER(u64)]),
@@ -272,6 +274,15 @@ class Context:
         t = np.ascontiguousarray(tiles, dtype=np.uint8).reshape(-1)
         r = Result()
         self._check(lib().ua_puzzle_solve(self.h, n, t.ctypes.data, C.byref(r)), (UA_OK, UA_NO_PATH))
+        return r.as_dict()
+
+    def puzzle_solve_bounded(self, n: int, tiles, mem_bytes: int) -> dict:
+        """The memory-bounded scheme: plain A* in mem_bytes, else weighted searches that
+        drop what does not fit; result["optimal"] says whether the cost is proven."""
+        t = np.ascontiguousarray(tiles, dtype=np.uint8).reshape(-1)
+        r = Result()
+        self._check(lib().ua_puzzle_solve_bounded(self.h, n, t.ctypes.data, mem_bytes, C.byref(r)),
+                    (UA_OK, UA_NO_PATH))
         return r.as_dict()
 
     def puzzle_moves(self) -> list:

@@ -83,8 +83,10 @@ enum : int32_t {
     ST_SUSPENDED = 2,   // round budget of a serving launch used up; state saved for resumption
 };
 // error bits raised by the kernels
-// FLAG_TIMEUP is no error: the launch's time budget ran out (serving suspends the query)
-enum : uint32_t { FLAG_CAPACITY = 1, FLAG_RANGE = 2, FLAG_ABORT = 4, FLAG_TIMEUP = 8 };
+// FLAG_TIMEUP is no error: the launch's time budget ran out (serving suspends the query).
+// FLAG_PRUNED is no error either: a memory-bounded search dropped successors that did not
+// fit its hash or heaps, so its answer is not proven optimal.
+enum : uint32_t { FLAG_CAPACITY = 1, FLAG_RANGE = 2, FLAG_ABORT = 4, FLAG_TIMEUP = 8, FLAG_PRUNED = 16 };
 
 // One search instance as the kernels see it. All pointers are device pointers.
 struct QueryDesc {
@@ -108,6 +110,11 @@ struct QueryDesc {
     uint64_t *resume;        // serving: resume record (resume_words(heaps) u64), or null
     int32_t round_budget;    // serving: at most this many rounds per launch (0 = no limit)
     int32_t cls;             // host side: instance class of the launch (CLS_*)
+    // puzzles, memory-bounded scheme: keys order by (g >> wshift) + h, i.e. weight
+    // 2^wshift on h, and the search stops at its first goal when wshift > 0; bounded:
+    // successors that do not fit the hash or the heaps are dropped (FLAG_PRUNED)
+    int32_t wshift;
+    int32_t bounded;
 };
 
 // Problem sets of the search kernel: SET_MAIN solves grids and 3x3 / 4x4 puzzles,
@@ -158,7 +165,7 @@ struct StepHeader {
 
 // Resume record of a suspended query: header, then per heap its root batch,
 // partial batch and (n, pc). Written when a serving launch's round budget runs out.
-constexpr int RESUME_HEADER = 8;   // valid, round, best_goal, pops, expanded, generated, inserted, spare
+constexpr int RESUME_HEADER = 8;   // valid, round, best_goal, pops, expanded, generated, inserted, flags
 constexpr int RESUME_PER_HEAP = 130;
 UA_HD inline int64_t resume_words(int heaps) { return RESUME_HEADER + (int64_t)heaps * RESUME_PER_HEAP; }
 

@@ -183,6 +183,8 @@ void fill_result(const QueryDesc &q, const QueryResult &r, ua_result *o) {
     o->generated = r.generated;
     o->inserted = r.inserted;
     o->device_ms = ticks_ms(r.t_start, r.t_end);
+    o->weight = 1 << q.wshift;
+    o->optimal = o->found && q.wshift == 0 && !(r.flags & FLAG_PRUNED);
 }
 
 // ---- plain solve (whole GPU, cooperative launch) ----------------------------------
@@ -445,6 +447,53 @@ extern "C" int ua_puzzle_solve(ua_ctx *c, int n, const uint8_t *tiles, ua_result
     fill_result(c->last_q, c->last_r, res);
     res->kernel_ms = c->last_kernel_ms;
     if (res->status < 0) fail(c, res->status, "puzzle solve failed with status %d", res->status);
+    return res->status;
+}
+
+extern "C" int ua_puzzle_solve_bounded(ua_ctx *c, int n, const uint8_t *tiles, uint64_t mem_bytes, ua_result *res) {
+    if (!c || !res || !tiles) return fail(c, UA_ERR_INVALID, "null argument");
+    c->last_valid = false;
+    std::memset(res, 0, sizeof *res);
+    if (!board_ok(n, tiles)) return fail(c, UA_ERR_INVALID, "board is not a permutation of 0..n*n-1");
+    if (mem_bytes < (16ull << 20)) return fail(c, UA_ERR_INVALID, "memory bound below 16 MiB");
+    if (!board_solvable(n, tiles)) {
+        res->status = UA_NO_PATH;
+        return UA_NO_PATH;
+    }
+    int rc = ensure_pdb(c, n);
+    if (rc) return rc;
+    // half the bound for the hash (keys and values), half for the heaps
+    const size_t slot_bytes = n == 5 ? 20 : 12;
+    int hl = 10;
+    while (hl < 32 && (slot_bytes << (hl + 1)) <= mem_bytes / 2) ++hl;
+    for (int ws = 0; ws <= 3; ++ws) {
+        // A round pops up to 64 keys a heap. The weighted searches run on fewer heaps, so
+        // that the hash lasts some 500 rounds of about 3 new boards a pop: a narrower
+        // frontier reaches deeper in the same memory.
+        int heaps = plain_heaps(c, KIND_PUZZLE);
+        if (ws > 0) {
+            int64_t want = ((1ll << hl) / (64 * 3 * 512)) / kBlockWaves * kBlockWaves;
+            heaps = (int)std::max<int64_t>(kBlockWaves, std::min<int64_t>(heaps, want));
+        }
+        const int64_t per_heap = (int64_t)(mem_bytes / 2 / 512) / heaps;  // 64-key nodes per heap
+        if (per_heap < 3) return fail(c, UA_ERR_INVALID, "memory bound too small for %d heaps", heaps);
+        QueryDesc q;
+        Plan p;
+        puzzle_fill_desc(c, n, tiles, heaps, hl, q, p);
+        q.heap_cap = (int)std::min<int64_t>(per_heap + 1, 1 << 30);
+        p.heap_bytes = (size_t)heaps * (size_t)(q.heap_cap - 1) * 512;
+        q.wshift = ws;
+        q.bounded = ws > 0;
+        rc = run_wide(c, q, p);
+        if (rc) return rc;
+        const QueryResult &r = c->last_r;
+        if (r.status == ST_OK && r.found) break;
+        if (ws == 0 && r.status != ST_CAPACITY && r.status != ST_RANGE) break;  // no path, or an error
+    }
+    fill_result(c->last_q, c->last_r, res);
+    res->kernel_ms = c->last_kernel_ms;
+    if (res->status == UA_NO_PATH && c->last_q.bounded) res->status = UA_ERR_CAPACITY;  // pruned away
+    if (res->status < 0) fail(c, res->status, "bounded puzzle solve failed with status %d", res->status);
     return res->status;
 }
 

@@ -428,7 +428,7 @@ struct PuzzleProb {
             }
             slot = (slot + 1) & hmask;
         }
-        err |= FLAG_CAPACITY;
+        err |= q.bounded ? FLAG_PRUNED : FLAG_CAPACITY;
         return SENT;
     }
     __device__ uint64_t seed(const Scope &sc) const {
@@ -438,8 +438,9 @@ struct PuzzleProb {
         uint64_t h = (uint64_t)puzzle_h(tb, q.start_state, pdb);
         return (h << 56) | (255ull << 48) | slot;
     }
+    // a weighted search (memory-bounded scheme) stops at its first goal
     __device__ bool goal_done(uint64_t goal, uint64_t lb) const {
-        return goal != SENT && (goal >> 32) <= (lb >> 56);
+        return goal != SENT && (q.wshift > 0 || (goal >> 32) <= (lb >> 56));
     }
     template <class SC>
     __device__ __forceinline__ void expand(const SC &sc, uint64_t it, uint64_t (&s)[MAXS], int &cnt,
@@ -470,8 +471,9 @@ struct PuzzleProb {
             uint64_t ns = b + (t << (4 * bl)) - (t << (4 * nb));
             uint32_t ng = gb + 1;
             int h = puzzle_h(tb, ns, pdb);
-            if (ng + h > 255u) {
-                err |= FLAG_RANGE;
+            const uint32_t f = (ng >> q.wshift) + (uint32_t)h;
+            if (f > 255u) {
+                err |= q.bounded ? FLAG_PRUNED : FLAG_RANGE;
                 continue;
             }
             uint64_t ns_slot = find_or_insert(sc, ns, err);
@@ -483,7 +485,7 @@ struct PuzzleProb {
                 uint64_t gk = ((uint64_t)ng << 32) | ns_slot;
                 goalmin = goalmin < gk ? goalmin : gk;
             } else {
-                s[d] = ((uint64_t)(ng + h) << 56) | ((uint64_t)(255u - ng) << 48) | ns_slot;
+                s[d] = ((uint64_t)f << 56) | ((uint64_t)(255u - ng) << 48) | ns_slot;
                 ++cnt;
             }
         }
@@ -537,11 +539,11 @@ __device__ __forceinline__ uint32_t rank5(uint64_t w0, uint64_t w1, int p) {
 __constant__ static const int8_t kPart5[25] = {-1, 1, 1, 0, 0, 0, 1, 2, 2, 2, 0, 1, 2,
                                                3, 3, 0, 1, 2, 3, 3, 0, 1, 2, 3, 3};
 
-// Item key and hash values as PuzzleProb; hash keys are two words per slot
-// (keys[2 slot], keys[2 slot + 1]). A slot is claimed by a CAS of its first word, and
+// Item key: (f << 56) | ((255 - g) << 48) | (h << 40) | slot; hash values as
+// PuzzleProb; hash keys are two words per slot (keys[2 slot], keys[2 slot + 1]). A slot is claimed by a CAS of its first word, and
 // the winner stores the second word at once; a prober that finds its first word in a
 // slot whose second word is not yet stored reads it again (a bounded wait). The child's
-// h is the parent's (f - g of its item) with the moved tile's partition term swapped.
+// h is the parent's (from its item) with the moved tile's partition term swapped.
 struct PuzzleProb5 {
     static constexpr int MAXS = 4;
     static constexpr int MAXPROBE = 256;
@@ -585,7 +587,7 @@ struct PuzzleProb5 {
             }
             slot = (slot + 1) & hmask;
         }
-        err |= FLAG_CAPACITY;
+        err |= q.bounded ? FLAG_PRUNED : FLAG_CAPACITY;
         return SENT;
     }
     __device__ uint64_t seed(const Scope &sc) const {
@@ -594,22 +596,23 @@ struct PuzzleProb5 {
         if (slot == SENT) return SENT;
         sc.store(&q.vals[slot], (uint32_t)15);
         uint64_t h = (uint64_t)h_full(q.start_state, q.start_state2);
-        return (h << 56) | (255ull << 48) | slot;
+        return (h << 56) | (255ull << 48) | (h << 40) | slot;
     }
+    // a weighted search (memory-bounded scheme) stops at its first goal
     __device__ bool goal_done(uint64_t goal, uint64_t lb) const {
-        return goal != SENT && (goal >> 32) <= (lb >> 56);
+        return goal != SENT && (q.wshift > 0 || (goal >> 32) <= (lb >> 56));
     }
     template <class SC>
     __device__ __forceinline__ void expand(const SC &sc, uint64_t it, uint64_t (&s)[MAXS], int &cnt,
                                            uint64_t &goalmin, uint32_t &err, int64_t &n_exp, int64_t &n_gen,
                                            int64_t &n_ins) const {
-        uint64_t slot = it & ((1ull << 48) - 1);
+        uint64_t slot = it & ((1ull << 40) - 1);
         if (slot > hmask) {  // never produced by a correct round: fail the query, touch nothing
             err |= FLAG_ABORT;
             return;
         }
         uint32_t gi = 255u - (uint32_t)((it >> 48) & 255u);
-        int hp = (int)(it >> 56) - (int)gi;
+        int hp = (int)((it >> 40) & 255u);
         uint64_t w0 = sc.load(&q.best[2 * slot]);
         uint64_t w1 = sc.load(&q.best[2 * slot + 1]);
         uint32_t v = sc.load(&q.vals[slot]);
@@ -634,8 +637,9 @@ struct PuzzleProb5 {
             const uint64_t off = (uint64_t)p * 127512000ull;  // PSpec<5>::OFF[p]
             int h = hp - (int)q.pdb[off + rank5(w0, w1, p)] + (int)q.pdb[off + rank5(n0, n1, p)];
             uint32_t ng = gb + 1;
-            if (ng + h > 255u) {
-                err |= FLAG_RANGE;
+            const uint32_t f = (ng >> q.wshift) + (uint32_t)h;
+            if (f > 255u) {
+                err |= q.bounded ? FLAG_PRUNED : FLAG_RANGE;
                 continue;
             }
             uint64_t ns_slot = find_or_insert(sc, n0, n1, err);
@@ -647,7 +651,7 @@ struct PuzzleProb5 {
                 uint64_t gk = ((uint64_t)ng << 32) | ns_slot;
                 goalmin = goalmin < gk ? goalmin : gk;
             } else {
-                s[d] = ((uint64_t)(ng + h) << 56) | ((uint64_t)(255u - ng) << 48) | ns_slot;
+                s[d] = ((uint64_t)f << 56) | ((uint64_t)(255u - ng) << 48) | ((uint64_t)h << 40) | ns_slot;
                 ++cnt;
             }
         }
@@ -705,6 +709,7 @@ __device__ void team_solve(const QueryDesc &q, QueryResult *res, const Team &tea
 
     int64_t n_pop = 0, n_exp = 0, n_gen = 0, n_ins = 0;
     uint64_t best_goal = resumed ? q.resume[2] : SENT;
+    uint32_t pruned = resumed ? (uint32_t)q.resume[7] & FLAG_PRUNED : 0u;
     int32_t status = ST_OK;
     int found = 0;
     const int r0 = resumed ? (int)q.resume[1] : 0;
@@ -769,7 +774,8 @@ __device__ void team_solve(const QueryDesc &q, QueryResult *res, const Team &tea
         const uint64_t lb = view->lb, g = view->goal;
         const uint32_t cnt = view->cnt, rerr = view->err;  // rewritten only after the next team barrier
         best_goal = best_goal < g ? best_goal : g;
-        if (rerr & ~(uint32_t)FLAG_TIMEUP) {
+        pruned |= rerr & FLAG_PRUNED;
+        if (rerr & ~(uint32_t)(FLAG_TIMEUP | FLAG_PRUNED)) {
             status = (rerr & FLAG_ABORT) ? ST_ABORT : (rerr & FLAG_CAPACITY) ? ST_CAPACITY : ST_RANGE;
             break;
         }
@@ -796,7 +802,7 @@ __device__ void team_solve(const QueryDesc &q, QueryResult *res, const Team &tea
             v = wave_sort(v);
             fits &= heap.insert(v, sreal);
         }
-        if (!fits && l == 0) sc.fold_or(&nxt->err, (uint32_t)FLAG_CAPACITY);
+        if (!fits && l == 0) sc.fold_or(&nxt->err, q.bounded ? (uint32_t)FLAG_PRUNED : (uint32_t)FLAG_CAPACITY);
         if (q.resume != nullptr && ((rerr & FLAG_TIMEUP) || (q.round_budget > 0 && r + 1 - r0 >= q.round_budget))) {
             status = ST_SUSPENDED;  // every wave reaches this with the same r and view
             break;
@@ -828,6 +834,7 @@ __device__ void team_solve(const QueryDesc &q, QueryResult *res, const Team &tea
             q.resume[1] = (uint64_t)(r + 1);
             q.resume[2] = best_goal;
             for (int i = 0; i < 4; ++i) q.resume[3 + i] = sc.load(&stats[i]);
+            q.resume[7] = pruned;
             q.resume[0] = 1;
         }
         res->status = status;
@@ -838,7 +845,7 @@ __device__ void team_solve(const QueryDesc &q, QueryResult *res, const Team &tea
         res->expanded = (int64_t)sc.load(&stats[1]);
         res->generated = (int64_t)sc.load(&stats[2]);
         res->inserted = (int64_t)sc.load(&stats[3]) + 1;
-        res->flags = 0;
+        res->flags = pruned;
     }
 }
 
