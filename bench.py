@@ -361,7 +361,7 @@ def run_serve_mixed(args):
         "tuning": tuning,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
-                     "kernel": "k_teams<2> (serving step)", "launches": launches,
+                     "kernel": "k_teams<2, 0> (serving step, grids and 15-puzzles)", "launches": launches,
                      "kernel_ms_avg": kms / max(1, launches), "alg_bytes_per_launch": alg / max(1, launches),
                      "kernel_ms_total": kms, "alg_bytes_total": alg},
     }
