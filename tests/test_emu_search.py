@@ -66,6 +66,7 @@ class Batch:
     def add_grid(self, graph, H, W, sx, sy, ex, ey, items=65536):
         bm = U.bytes_to_bitmap(graph, H, W)
         best = np.full(H * W, np.uint64(~np.uint64(0)))
+        best[np.asarray(graph).reshape(-1) == 0] ^= np.uint64(16)  # each word carries its cell's open bit
         per_heap = (items + 64 * self.waves - 1) // (64 * self.waves)
         cap = max(per_heap + 1, 4)
         heap = np.zeros(self.waves * (cap - 1) * 64, np.uint64)

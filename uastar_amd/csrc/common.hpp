@@ -64,6 +64,12 @@ UA_HD inline void key_to_pair(uint64_t key, int64_t *a, int64_t *b) {
     *a = (int64_t)((key - bb * KEY_SQRT2) >> 36);
 }
 
+// Grid best word, one per cell: (g_key << kGridKeyShift) | (open << 4) | dir, where dir
+// is the move that reached the cell (8 = start, 15 = unseen). All ones with the open bit
+// as the cell's is "unseen". g_key < KEY_LIMIT < 2^54, so the word never overflows.
+constexpr int kGridKeyShift = 5;
+constexpr uint64_t kOpenBit = 16;
+
 // ------------------------------------------------------------------ query records
 enum : int32_t { KIND_GRID = 1, KIND_PUZZLE = 2 };
 
@@ -101,7 +107,7 @@ struct QueryDesc {
     int32_t max_rounds;
     int32_t heaps;           // heaps (= waves) in the team; filled by the launcher
     const uint64_t *bitmap;  // grid: row-padded, 1 = open
-    uint64_t *best;          // grid: per cell (gkey << 4 | dir);  puzzle: hash keys
+    uint64_t *best;          // grid: per cell (gkey << 5 | open << 4 | dir);  puzzle: hash keys
     uint32_t *vals;          // puzzle: hash values (g << 4 | dir)
     uint64_t *heap;          // heap node storage, heaps * (heap_cap-1) * 64 u64
     uint64_t start_state;    // puzzle board (n = 5: its first word)

@@ -131,6 +131,15 @@ __global__ void k_grid_gen(uint64_t *bm, int H, int W, int wpr, uint64_t seed_mi
     bm[w] = word;
 }
 
+// Clears the open bit in the best word of every blocked cell of a table just filled
+// with all ones, so that each word carries its cell's open bit. One thread per cell.
+__global__ void k_grid_mark_blocked(uint64_t *best, const uint64_t *bm, int H, int W, int wpr) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)H * W) return;
+    int x = (int)(i / (unsigned)W), y = (int)(i - (size_t)x * W);
+    if (!((bm[(size_t)x * wpr + (y >> 6)] >> (y & 63)) & 1ull)) best[i] = ~kOpenBit;
+}
+
 // Byte-per-cell input (pathway.hpp:116-119, 0xFF open / 0 blocked) to the bitmap.
 __global__ void k_grid_pack(const uint8_t *bytes, uint64_t *bm, int H, int W, int wpr) {
     size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -265,6 +274,12 @@ hipError_t launch_grid_gen(uint64_t *bm, int H, int W, int wpr, uint64_t seed, u
 hipError_t launch_grid_pack(const uint8_t *bytes, uint64_t *bm, int H, int W, int wpr, hipStream_t st) {
     size_t n = (size_t)H * wpr;
     hipLaunchKernelGGL(k_grid_pack, dim3(grid_blocks(n, 256)), dim3(256), 0, st, bytes, bm, H, W, wpr);
+    return hipGetLastError();
+}
+
+hipError_t launch_grid_mark_blocked(uint64_t *best, const uint64_t *bm, int H, int W, int wpr, hipStream_t st) {
+    size_t n = (size_t)H * W;
+    hipLaunchKernelGGL(k_grid_mark_blocked, dim3(grid_blocks(n, 256)), dim3(256), 0, st, best, bm, H, W, wpr);
     return hipGetLastError();
 }
 

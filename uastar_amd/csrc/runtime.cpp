@@ -222,6 +222,7 @@ static int run_wide(ua_ctx *c, QueryDesc &q, const Plan &p) {
     q.heap = c->heap.as<uint64_t>();
     if (q.kind == KIND_GRID) {
         HIPCHK(c, hipMemsetAsync(q.best, 0xFF, p.best_bytes, c->st));
+        HIPCHK(c, launch_grid_mark_blocked(q.best, q.bitmap, q.H, q.W, q.wpr, c->st));
     } else {
         HIPCHK(c, hipMemsetAsync(q.best, 0, p.best_bytes, c->st));
         HIPCHK(c, hipMemsetAsync(q.vals, 0xFF, p.vals_bytes, c->st));
@@ -754,6 +755,8 @@ extern "C" int ua_batch_solve(ua_ctx *c, const ua_query *qs, int64_t nq, ua_resu
             }
             HIPCHK(c, hipMemsetAsync(base, 0xFF, ff_end, c->st));
             if (zero_end > ff_end) HIPCHK(c, hipMemsetAsync(base + ff_end, 0, zero_end - ff_end, c->st));
+            for (const QueryDesc &q : sub)
+                if (q.kind == KIND_GRID) HIPCHK(c, launch_grid_mark_blocked(q.best, q.bitmap, q.H, q.W, q.wpr, c->st));
             std::vector<QueryResult> sr;
             int rc = run_batch(c, sub, sr, 1);
             if (rc) return rc;

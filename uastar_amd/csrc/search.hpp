@@ -174,17 +174,15 @@ struct GridProb {
         ymask = (1ull << q.ybits) - 1;
         goal_id = (uint64_t)q.ex * (uint64_t)q.W + (uint64_t)q.ey;
     }
-    __device__ bool open(int x, int y) const {
-        return (q.bitmap[(size_t)x * q.wpr + (y >> 6)] >> (y & 63)) & 1ull;
-    }
     __device__ uint64_t item(uint64_t fkey, int x, int y) const {
         return ((fkey >> q.qshift) << q.cbits) | ((uint64_t)x << q.ybits) | (uint64_t)y;
     }
     __device__ bool trivial() const { return q.sx == q.ex && q.sy == q.ey; }
-    // Called by one lane: writes the start record and returns its item key.
+    // Called by one lane: writes the start record (g 0, direction 8, the cell's open bit
+    // kept) and returns its item key.
     __device__ uint64_t seed(const Scope &sc) const {
         uint64_t id = (uint64_t)q.sx * q.W + q.sy;
-        sc.store(&q.best[id], (uint64_t)8);
+        sc.store(&q.best[id], (uint64_t)((sc.load(&q.best[id]) & kOpenBit) | 8u));
         return item(octile_key(q.sx, q.sy, q.ex, q.ey), q.sx, q.sy);
     }
     __device__ bool goal_done(uint64_t goal, uint64_t lb) const {
@@ -198,10 +196,11 @@ struct GridProb {
     __device__ __forceinline__ static uint32_t uabs(int v) { return (uint32_t)(v < 0 ? -v : v); }
 
     // Per-lane expansion of one popped item; successors land in s[0..7] (SENT = none).
-    // Memory is touched in three waves of independent accesses: the cell's best word and
-    // the 3x3 bitmap window around it (six words); the legal neighbours' best words; an
-    // atomicMin for each neighbour the move would improve. Coordinates and offsets are
-    // 32-bit; a neighbour's best word is addressed from the cell's (row offsets +-W).
+    // A best word is (g_key << 5) | (open << 4) | dir: it carries its cell's open bit, so
+    // the expansion needs no bitmap. Memory is touched in three waves of independent
+    // accesses: the cell's best word; every in-range neighbour's best word; an atomicMin
+    // for each open neighbour (or the goal) the move would improve. Coordinates and
+    // offsets are 32-bit; a neighbour's word is addressed from the cell's (rows +-W).
     template <class SC>
     __device__ __forceinline__ void expand(const SC &sc, uint64_t it, uint64_t (&s)[MAXS], int &cnt,
                                            uint64_t &goalmin, uint32_t &err, int64_t &n_exp, int64_t &n_gen,
@@ -213,34 +212,12 @@ struct GridProb {
             return;
         }
         uint64_t *const bp = q.best + ((uint64_t)(uint32_t)x * (uint32_t)q.W + (uint32_t)y);
-        // issue every load first
         const uint64_t bw = sc.load(bp);
-        const int clo = y > 0 ? y - 1 : 0, chi = y + 1 < q.W ? y + 1 : q.W - 1;
-        uint64_t wlo[3], whi[3];
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-            int r = x + i - 1;
-            r = r < 0 ? 0 : (r >= q.H ? q.H - 1 : r);
-            const uint64_t *row = q.bitmap + (uint64_t)(uint32_t)r * (uint32_t)q.wpr;
-            wlo[i] = row[clo >> 6];
-            whi[i] = row[chi >> 6];
-        }
         const int gx = x - q.ex, gy = y - q.ey;  // offsets to the goal
-        const uint64_t gk = bw >> 4;
+        const uint64_t gk = bw >> kGridKeyShift;
         if (((gk + oct(uabs(gx), uabs(gy))) >> q.qshift) != (it >> q.cbits)) return;  // stale
         ++n_exp;
-        // window bit (3 * i + j) = cell (x + i - 1, y + j - 1) is in range and open
-        uint32_t win = 0;
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-            if ((uint32_t)(x + i - 1) >= (uint32_t)q.H) continue;
-            const uint64_t wc = (y & 63) == 0 ? whi[i] : wlo[i];  // the word of column y
-            win |= (uint32_t)(y > 0 && ((wlo[i] >> ((y - 1) & 63)) & 1ull)) << (3 * i);
-            win |= (uint32_t)((wc >> (y & 63)) & 1ull) << (3 * i + 1);
-            win |= (uint32_t)(y + 1 < q.W && ((whi[i] >> ((y + 1) & 63)) & 1ull)) << (3 * i + 2);
-        }
-        if (!((win >> 4) & 1u)) return;  // source-cell mask rule
-        // legal moves: in range, and open unless the goal (a blocked cell is a dead end)
+        if (!(bw & kOpenBit)) return;  // source-cell mask rule: a blocked cell has no moves
         const int64_t Wl = q.W;
         uint64_t w[8];
         uint32_t live = 0;
@@ -248,30 +225,32 @@ struct GridProb {
         for (int d = 0; d < 8; ++d) {
             const int nx = x + kDX[d], ny = y + kDY[d];
             if ((uint32_t)nx >= (uint32_t)q.H || (uint32_t)ny >= (uint32_t)q.W) continue;
-            const bool is_goal = gx + kDX[d] == 0 && gy + kDY[d] == 0;
-            if (!is_goal && !((win >> (3 * (kDX[d] + 1) + (kDY[d] + 1))) & 1u)) continue;
-            ++n_gen;
-            const uint64_t ng = gk + (d < 4 ? KEY_ONE : KEY_SQRT2);
-            if (ng + oct(uabs(gx + kDX[d]), uabs(gy + kDY[d])) >= KEY_LIMIT) {
-                err |= FLAG_RANGE;
-                continue;
-            }
             live |= 1u << d;
             w[d] = sc.load(bp + (kDX[d] * Wl + kDY[d]));
         }
-        // read, then relax with an atomic only where the new cost is lower than what was read
-        // (best words only ever decrease, so a skipped atomic could not have succeeded)
+        // legal moves: open targets, and the goal even if blocked (any other blocked cell
+        // is a dead end); relax with an atomic only where the new cost is lower than what
+        // was read (best words only ever decrease, so a skipped atomic could not succeed)
 #pragma unroll
         for (int d = 0; d < 8; ++d) {
+            if (!((live >> d) & 1u)) continue;
+            const bool is_goal = gx + kDX[d] == 0 && gy + kDY[d] == 0;
+            if (!is_goal && !(w[d] & kOpenBit)) continue;
+            ++n_gen;
             const uint64_t ng = gk + (d < 4 ? KEY_ONE : KEY_SQRT2);
-            if (!((live >> d) & 1u) || (w[d] >> 4) <= ng) continue;
-            const uint64_t old = sc.fmin(bp + (kDX[d] * Wl + kDY[d]), (ng << 4) | (uint64_t)d);
-            if ((old >> 4) <= ng) continue;
+            const uint64_t nf = ng + oct(uabs(gx + kDX[d]), uabs(gy + kDY[d]));
+            if (nf >= KEY_LIMIT) {
+                err |= FLAG_RANGE;
+                continue;
+            }
+            if ((w[d] >> kGridKeyShift) <= ng) continue;
+            const uint64_t old =
+                sc.fmin(bp + (kDX[d] * Wl + kDY[d]), (ng << kGridKeyShift) | (w[d] & kOpenBit) | (uint64_t)d);
+            if ((old >> kGridKeyShift) <= ng) continue;
             ++n_ins;
-            if (gx + kDX[d] == 0 && gy + kDY[d] == 0) {
+            if (is_goal) {
                 goalmin = goalmin < ng ? goalmin : ng;
             } else {
-                const uint64_t nf = ng + oct(uabs(gx + kDX[d]), uabs(gy + kDY[d]));
                 s[d] = ((nf >> q.qshift) << q.cbits) | ((uint64_t)(uint32_t)(x + kDX[d]) << q.ybits) |
                        (uint64_t)(uint32_t)(y + kDY[d]);
                 ++cnt;
@@ -851,7 +830,7 @@ __device__ void team_solve(const QueryDesc &q, QueryResult *res, const Team &tea
 
 // A query of the other problem set (set_of) fails with ST_INVALID.
 template <int SET, class Team>
-__device__ void dispatch(const QueryDesc &q, QueryResult *res, const Team &team) {
+__device__ __forceinline__ void dispatch(const QueryDesc &q, QueryResult *res, const Team &team) {
     if (set_of(q) != SET) {
         if (team.leader()) {
             res->status = ST_INVALID;
@@ -878,7 +857,7 @@ __device__ void dispatch(const QueryDesc &q, QueryResult *res, const Team &team)
 // With a time budget the group's first hook fixes its deadline; after the deadline it
 // takes no more queries, and the round barrier of the query it runs flags FLAG_TIMEUP.
 template <int SET, class Team>
-__device__ void group_loop(const QueryDesc *qs, QueryResult *rs, const Team &team, TeamVars *tv, uint32_t *head,
+__device__ __forceinline__ void group_loop(const QueryDesc *qs, QueryResult *rs, const Team &team, TeamVars *tv, uint32_t *head,
                            int nq, int qoff, int64_t budget) {
     for (;;) {
         int64_t t0 = now_ticks();

@@ -162,9 +162,11 @@ int plan_job(ua_ctx *c, ServeEngine *e, Job &j, bool *fits) {
     j.d.heap = reinterpret_cast<uint64_t *>(base + o);
     o += align256(j.plan.heap_bytes);
     j.d.resume = reinterpret_cast<uint64_t *>(base + o);
-    // fresh search state: grid best = all ones; puzzle keys = 0, vals = all ones; no resume record
+    // fresh search state: grid best = all ones but blocked cells' open bit; puzzle keys = 0,
+    // vals = all ones; no resume record
     if (j.d.kind == KIND_GRID) {
         HIPCHK(c, hipMemsetAsync(j.d.best, 0xFF, j.plan.best_bytes, c->st));
+        HIPCHK(c, launch_grid_mark_blocked(j.d.best, j.d.bitmap, j.d.H, j.d.W, j.d.wpr, c->st));
     } else {
         HIPCHK(c, hipMemsetAsync(j.d.best, 0, j.plan.best_bytes, c->st));
         HIPCHK(c, hipMemsetAsync(j.d.vals, 0xFF, j.plan.vals_bytes, c->st));
