@@ -120,11 +120,18 @@ struct GroupTeam {
     // hook, resets the counter and bumps the generation. A bounded spin turns a lost
     // block into an abort that every block of the group observes, never a hang.
     // A one-block group is one workgroup barrier around the hook.
+    //
+    // Across blocks, agent-scope ordering costs an L2 write-back (release) or an L1/L2
+    // invalidation (acquire) on this XCD, which also evicts the data of the other
+    // blocks there. So only thread 0 orders, once each way per barrier, around relaxed
+    // atomics (a release fence before the arrival, an acquire fence after the
+    // generation moves); the workgroup barriers carry the block's other threads. The
+    // spin itself reads relaxed.
     template <class Hook>
     __device__ bool sync(Hook hook) const {
-        sc.release();
-        __syncthreads();
         if (bpg == 1) {
+            sc.release();
+            __syncthreads();
             if (threadIdx.x == 0) hook();
             sc.release();
             __syncthreads();
@@ -133,17 +140,21 @@ struct GroupTeam {
         }
         constexpr int SC = __HIP_MEMORY_SCOPE_AGENT;
         __shared__ uint32_t s_ok;
+        __syncthreads();
         if (threadIdx.x == 0) {
             uint32_t ok = 1;
-            uint32_t g = __hip_atomic_load(&tv->bar_gen, __ATOMIC_ACQUIRE, SC);
-            uint32_t a = __hip_atomic_fetch_add(&tv->bar_count, 1u, __ATOMIC_ACQ_REL, SC);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // the block's writes, before arriving
+            uint32_t g = __hip_atomic_load(&tv->bar_gen, __ATOMIC_RELAXED, SC);
+            uint32_t a = __hip_atomic_fetch_add(&tv->bar_count, 1u, __ATOMIC_RELAXED, SC);
             if (a == (uint32_t)bpg - 1) {
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // the others' writes, for the hook
                 hook();
                 __hip_atomic_store(&tv->bar_count, 0u, __ATOMIC_RELAXED, SC);
-                __hip_atomic_fetch_add(&tv->bar_gen, 1u, __ATOMIC_RELEASE, SC);
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // the hook's writes
+                __hip_atomic_fetch_add(&tv->bar_gen, 1u, __ATOMIC_RELAXED, SC);
             } else {
                 uint32_t spins = 0;
-                while (__hip_atomic_load(&tv->bar_gen, __ATOMIC_ACQUIRE, SC) == g) {
+                while (__hip_atomic_load(&tv->bar_gen, __ATOMIC_RELAXED, SC) == g) {
                     __builtin_amdgcn_s_sleep(1);
                     if (++spins > SPIN_LIMIT || __hip_atomic_load(&tv->abort, __ATOMIC_RELAXED, SC)) {
                         __hip_atomic_store(&tv->abort, 1u, __ATOMIC_RELAXED, SC);
@@ -153,17 +164,17 @@ struct GroupTeam {
                 }
             }
             if (__hip_atomic_load(&tv->abort, __ATOMIC_RELAXED, SC)) ok = 0;
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // everyone's writes, for the whole block
             s_ok = ok;
         }
         __syncthreads();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         return s_ok != 0;
     }
 };
 
 // ------------------------------------------------------------------ grid problem
 // Item key: (fq << cbits) | (x << ybits | y), fq = f_key >> qshift.
-// Best word per cell: (g_key << 4) | dir, dir = move into the cell (8 = start).
+// Best word per cell: (g_key << 5) | (open << 4) | dir, dir = move into the cell (8 = start).
 struct GridProb {
     static constexpr int MAXS = 8;
     QueryDesc q;
