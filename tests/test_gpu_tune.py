@@ -37,8 +37,8 @@ def test_autotune_then_solve(oracle):
         assert t["grid_team_waves"] in (16, 32) and t["grid_large_team_waves"] in (16, 32, 64)
         assert t["puzzle_team_waves"] in (16, 32, 64, 128)
         assert 18 <= t["puzzle_hash_log2"] <= 26 and 18 <= t["plain_hash_log2"] <= 26
-        assert t["plain_grid_heaps"] in (256, 512, 1024, 2048, 4096)
-        assert t["plain_puzzle_heaps"] in (256, 512, 1024, 2048, 4096)
+        assert t["plain_grid_heaps"] in (16, 32, 64, 128, 256, 512, 1024, 2048, 4096)
+        assert t["plain_puzzle_heaps"] in (64, 128, 256, 512, 1024, 2048, 4096)
         # every sample grid is below the large-grid threshold: small grids and puzzles are measured
         assert t["trials"] >= 2 + 4 + 2 and t["est_grid_ms"] > 0 and t["est_puzzle_ms"] > 0
         assert t["est_grid_large_ms"] == 0

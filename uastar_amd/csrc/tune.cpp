@@ -159,7 +159,9 @@ extern "C" int ua_autotune(ua_ctx *c, const ua_query *qs, int64_t n, ua_tuning *
             const size_t m = std::min<size_t>(3, src.size());
             int best_h = 0;
             double best = 0, best_ins = 0;
-            for (int h = 256; h <= std::min(4096, max_heaps); h *= 2) {
+            // depth-bound grids can run best on one block (fewer, fuller heaps); frontier-
+            // bound puzzles want many heaps
+            for (int h = kind == KIND_GRID ? kBlockWaves : 64; h <= std::min(4096, max_heaps); h *= 2) {
                 c->tune.plain_heaps[kind == KIND_GRID ? 0 : 1] = h;
                 double ms = 0, ins = 0;
                 bool ok = true;
