@@ -122,14 +122,15 @@ struct WaveHeap {
     }
     __device__ __forceinline__ int size() const { return n * 64 + pc; }
 
-    // Restore (H1) below the root after the root batch changed.
-    __device__ void sift_down() {
+    // Restore (H1) below the root after the root batch changed. The children of the
+    // node the batch descends to are loaded before this level's merges, so a level costs
+    // about one load latency rather than a latency plus the merges.
+    // L and R are the root's children (nodes 2 and 3, SENT where absent), loaded by the caller.
+    __device__ void sift_down(uint64_t L, uint64_t R) {
         int v = 1;
         uint64_t X = root;
-        while (true) {
-            int l = 2 * v;
-            if (l > n) break;
-            uint64_t L = load(l);
+        int l = 2;
+        while (l <= n) {
             uint64_t xmax = readlane64(X, 63);
             if (l + 1 > n) {
                 if (xmax <= readlane64(L, 0)) break;
@@ -137,33 +138,41 @@ struct WaveHeap {
                 store(l, L);
                 break;
             }
-            uint64_t R = load(l + 1);
             if (xmax <= readlane64(L, 0) && xmax <= readlane64(R, 0)) break;
             uint64_t lmax = readlane64(L, 63), rmax = readlane64(R, 63);
-            wave_merge(L, R);                       // L = smaller half, R = larger half
             int big = (lmax >= rmax) ? l : l + 1;   // larger half replaces the child with the larger max
             int small = (big == l) ? l + 1 : l;
+            const int nl = 2 * small;                // the next level: children of `small`
+            const uint64_t NL = nl <= n ? load(nl) : SENT;
+            const uint64_t NR = nl + 1 <= n ? load(nl + 1) : SENT;
+            wave_merge(L, R);                        // L = smaller half, R = larger half
             store(big, R);
-            wave_merge(X, L);                       // X keeps the 64 smallest
+            wave_merge(X, L);                        // X keeps the 64 smallest
             if (v == 1) root = X;
             else store(v, X);
             v = small;
             X = L;
+            l = nl;
+            L = NL;
+            R = NR;
         }
         if (v == 1) root = X;
         else store(v, X);
     }
 
-    // Pops the (up to) 64 smallest keys, one per lane, ascending; SENT where empty.
+    // Pops the (up to) 64 smallest keys, one per lane, ascending; SENT where empty. The
+    // root's children that remain after the last node moves up are loaded with it.
     __device__ uint64_t pop() {
         uint64_t out;
         if (n >= 1) {
             out = root;
             if (n >= 2) {
+                const uint64_t L = n >= 3 ? load(2) : SENT;
+                const uint64_t R = n >= 4 ? load(3) : SENT;
                 root = load(n);
                 --n;
                 wave_merge(root, part);  // root takes 64 of (last node U partial), partial keeps pc
-                sift_down();
+                sift_down(L, R);
             } else {
                 n = 0;
                 root = SENT;
