@@ -7,7 +7,7 @@
 //   (H1) for every node v and child c: max(v) <= min(c)
 //   (H2) every key of the partial batch is >= max(root)
 // so the root holds the 64 smallest keys of the heap and a pop returns them.
-// All merges are wave-wide bitonic networks over registers (ds_bpermute).
+// All merges are wave-wide bitonic networks over registers (DPP and permlane swaps).
 // This replaces uAstar's one-heap-per-thread sift-down/sift-up in global memory
 // (pathway/GPU-kernel.cuh:209-252 and :517-535; puzzle/GPU-kernel.cuh:262-298, :435-453).
 #pragma once
@@ -25,11 +25,47 @@ __device__ __forceinline__ uint64_t shfl64(uint64_t v, int src) {
     int hi = __shfl((int)(uint32_t)(v >> 32), src);
     return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
 }
+#ifdef UA_EMU
 __device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int m) {
     int lo = __shfl_xor((int)(uint32_t)v, m);
     int hi = __shfl_xor((int)(uint32_t)(v >> 32), m);
     return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
 }
+#else
+// Lane exchange l <-> l ^ m of one dword without the LDS crossbar (m a compile-time
+// constant after unrolling): DPP within rows of 16 lanes, the gfx950 permlane swaps
+// across rows and halves.
+//   m = 1, 2: quad_perm [1,0,3,2] / [2,3,0,1]
+//   m = 4:    row_half_mirror (l -> 7 - l within 8), then quad_perm [3,2,1,0]
+//   m = 8:    row_mirror (l -> 15 - l within 16), then row_half_mirror
+//   m = 16:   permlane16_swap of v with itself: the first result holds rows (0,0,2,2),
+//             the second rows (1,1,3,3)
+//   m = 32:   permlane32_swap of v with itself: the first holds halves (0,0), the second (1,1)
+__device__ __forceinline__ uint32_t xor_lanes32(uint32_t v, int m) {
+    const int x = (int)v;
+    switch (m) {
+    case 1: return (uint32_t)__builtin_amdgcn_mov_dpp(x, 0xB1, 0xF, 0xF, false);
+    case 2: return (uint32_t)__builtin_amdgcn_mov_dpp(x, 0x4E, 0xF, 0xF, false);
+    case 4: return (uint32_t)__builtin_amdgcn_mov_dpp(__builtin_amdgcn_mov_dpp(x, 0x141, 0xF, 0xF, false), 0x1B, 0xF,
+                                                      0xF, false);
+    case 8: return (uint32_t)__builtin_amdgcn_mov_dpp(__builtin_amdgcn_mov_dpp(x, 0x140, 0xF, 0xF, false), 0x141, 0xF,
+                                                      0xF, false);
+    case 16: {
+        auto p = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+        return (__lane_id() & 16) ? p[0] : p[1];
+    }
+    case 32: {
+        auto p = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+        return (__lane_id() & 32) ? p[0] : p[1];
+    }
+    default: return (uint32_t)__shfl_xor(x, m);
+    }
+}
+__device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int m) {
+    const uint32_t lo = xor_lanes32((uint32_t)v, m), hi = xor_lanes32((uint32_t)(v >> 32), m);
+    return ((uint64_t)hi << 32) | lo;
+}
+#endif
 __device__ __forceinline__ uint64_t readlane64(uint64_t v, int lane) {
     uint32_t lo = __builtin_amdgcn_readlane((int)(uint32_t)v, lane);
     uint32_t hi = __builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), lane);
