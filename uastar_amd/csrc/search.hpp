@@ -55,6 +55,15 @@ struct Scope {
         return wg ? __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
                   : __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
+    // A read that may return an older value within the round (workgroup scope: served
+    // by this CU's caches, which the team barrier's acquire refreshes). For the
+    // monotone search state only -- best words and hash values only decrease, a hash
+    // key changes once from 0 -- where an older value costs at most a failing atomic or
+    // an expansion with a cost that is no longer the best, never a wrong answer.
+    template <class T>
+    __device__ __forceinline__ T load_weak(const T *p) const {
+        return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
     template <class T>
     __device__ __forceinline__ void store(T *p, T v) const {
         if (wg) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -223,7 +232,7 @@ struct GridProb {
             return;
         }
         uint64_t *const bp = q.best + ((uint64_t)(uint32_t)x * (uint32_t)q.W + (uint32_t)y);
-        const uint64_t bw = sc.load(bp);
+        const uint64_t bw = sc.load_weak(bp);
         const int gx = x - q.ex, gy = y - q.ey;  // offsets to the goal
         const uint64_t gk = bw >> kGridKeyShift;
         if (((gk + oct(uabs(gx), uabs(gy))) >> q.qshift) != (it >> q.cbits)) return;  // stale
@@ -237,7 +246,7 @@ struct GridProb {
             const int nx = x + kDX[d], ny = y + kDY[d];
             if ((uint32_t)nx >= (uint32_t)q.H || (uint32_t)ny >= (uint32_t)q.W) continue;
             live |= 1u << d;
-            w[d] = sc.load(bp + (kDX[d] * Wl + kDY[d]));
+            w[d] = sc.load_weak(bp + (kDX[d] * Wl + kDY[d]));
         }
         // legal moves: open targets, and the goal even if blocked (any other blocked cell
         // is a dead end); relax with an atomic only where the new cost is lower than what
@@ -410,7 +419,7 @@ struct PuzzleProb {
     __device__ __forceinline__ uint64_t find_or_insert(const SC &sc, uint64_t b, uint32_t &err) const {
         uint64_t slot = mix64(b) & hmask;
         for (int i = 0; i < MAXPROBE; ++i) {
-            uint64_t k = sc.load(&q.best[slot]);
+            uint64_t k = sc.load_weak(&q.best[slot]);
             if (k == b) return slot;
             if (k == 0) {
                 uint64_t exp = 0;
@@ -442,8 +451,8 @@ struct PuzzleProb {
             return;
         }
         uint32_t gi = 255u - (uint32_t)((it >> 48) & 255u);
-        uint64_t b = sc.load(&q.best[slot]);
-        uint32_t v = sc.load(&q.vals[slot]);
+        uint64_t b = sc.load_weak(&q.best[slot]);
+        uint32_t v = sc.load_weak(&q.vals[slot]);
         uint32_t gb = v >> 4;
         if (gb != gi) return;  // stale
         ++n_exp;
@@ -557,7 +566,7 @@ struct PuzzleProb5 {
     __device__ __forceinline__ uint64_t find_or_insert(const SC &sc, uint64_t b0, uint64_t b1, uint32_t &err) const {
         uint64_t slot = mix64(b0 ^ mix64(b1)) & hmask;
         for (int i = 0; i < MAXPROBE; ++i) {
-            uint64_t k0 = sc.load(&q.best[2 * slot]);
+            uint64_t k0 = sc.load_weak(&q.best[2 * slot]);
             if (k0 == 0) {
                 uint64_t exp = 0;
                 if (sc.cas(&q.best[2 * slot], exp, b0)) {
@@ -603,9 +612,9 @@ struct PuzzleProb5 {
         }
         uint32_t gi = 255u - (uint32_t)((it >> 48) & 255u);
         int hp = (int)((it >> 40) & 255u);
-        uint64_t w0 = sc.load(&q.best[2 * slot]);
-        uint64_t w1 = sc.load(&q.best[2 * slot + 1]);
-        uint32_t v = sc.load(&q.vals[slot]);
+        uint64_t w0 = sc.load_weak(&q.best[2 * slot]);
+        uint64_t w1 = sc.load_weak(&q.best[2 * slot + 1]);
+        uint32_t v = sc.load_weak(&q.vals[slot]);
         uint32_t gb = v >> 4;
         if (gb != gi) return;  // stale
         ++n_exp;
