@@ -50,3 +50,39 @@ extern "C" void emu_move5(uint64_t *w0, uint64_t *w1, int bl, int nb) {
     ua::xor5(*w0, *w1, bl, t);
     ua::xor5(*w0, *w1, nb, t);
 }
+
+// The device heap self-test (k_heap_selftest) on one emulated wave: ops[i] >= 0 inserts
+// keys[k .. k + ops[i]), ops[i] < 0 pops 64 keys into out; *status = final size or -1.
+extern "C" int emu_heap_selftest(const int *ops, int nops, const uint64_t *keys, uint64_t *storage, int cap,
+                                 uint64_t *out, int *status) {
+    std::barrier<> bar(64);
+    emu::Wave w;
+    emu::Block blk{&bar, &w, 64u};
+    emu::block = &blk;
+    std::vector<std::thread> th;
+    for (int t = 0; t < 64; ++t)
+        th.emplace_back([&, t] {
+            emu::tid = (unsigned)t;
+            ua::WaveHeap h;
+            h.init(storage, cap);
+            const int l = ua::lane_id();
+            int k = 0, o = 0;
+            bool ok = true;
+            for (int i = 0; i < nops; ++i) {
+                const int c = ops[i];
+                if (c >= 0) {
+                    uint64_t v = l < c ? keys[k + l] : ua::SENT;
+                    k += c;
+                    v = ua::wave_sort(v);
+                    ok &= h.insert(v, c);
+                } else {
+                    out[(size_t)o * 64 + l] = h.pop();
+                    ++o;
+                }
+            }
+            if (l == 0) *status = ok ? h.size() : -1;
+        });
+    for (auto &x : th) x.join();
+    emu::block = nullptr;
+    return 0;
+}

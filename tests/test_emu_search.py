@@ -303,3 +303,37 @@ def test_emulated_bounded_weighted_puzzles(emu, oracle):
     for r in res[3:5]:  # bounded: pruned, no error
         assert r.status in (0, 1) and r.flags & 16
     assert res[5].status == -3  # unbounded: capacity error
+
+
+def test_emulated_heap_selftest(emu):
+    """The per-wave batched heap on one emulated wave: random insert batches (0..64 keys,
+    narrow and wide key ranges, so ties are common) and pops; every pop returns the 64
+    smallest keys present, in order."""
+    emu.emu_heap_selftest.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                                      C.POINTER(C.c_int)]
+    rng = np.random.default_rng(17)
+    for trial in range(2):
+        ops, keys, ref, pops = [], [], [], []
+        for _ in range(110):
+            if rng.random() < 0.7:
+                c = int(rng.integers(0, 65))
+                ks = rng.integers(0, [10, 1 << 62][trial], size=c, dtype=np.uint64)
+                ops.append(c)
+                keys.extend(ks.tolist())
+                ref.extend(ks.tolist())
+            else:
+                ops.append(-1)
+                ref.sort()
+                pops.append(ref[:64])
+                del ref[:64]
+        ops_a = np.array(ops, np.int32)
+        keys_a = np.array(keys, np.uint64)
+        storage = np.zeros(4096 * 64, np.uint64)
+        out = np.zeros(max(1, len(pops)) * 64, np.uint64)
+        status = C.c_int()
+        assert emu.emu_heap_selftest(ops_a.ctypes.data, len(ops), keys_a.ctypes.data, storage.ctypes.data, 4096,
+                                     out.ctypes.data, C.byref(status)) == 0
+        assert status.value == len(ref), trial
+        for i, exp in enumerate(pops):
+            row = [int(v) for v in out[i * 64:(i + 1) * 64] if int(v) != (1 << 64) - 1]
+            assert row == [int(v) for v in exp], f"trial {trial} pop {i}"

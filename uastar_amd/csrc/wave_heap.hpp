@@ -158,57 +158,65 @@ struct WaveHeap {
     }
     __device__ __forceinline__ int size() const { return n * 64 + pc; }
 
-    // Restore (H1) below the root after the root batch changed. The children of the
-    // node the batch descends to are loaded before this level's merges, so a level costs
-    // about one load latency rather than a latency plus the merges.
-    // L and R are the root's children (nodes 2 and 3, SENT where absent), loaded by the caller.
-    __device__ void sift_down(uint64_t L, uint64_t R) {
-        int v = 1;
-        uint64_t X = root;
-        int l = 2;
-        while (l <= n) {
-            uint64_t xmax = readlane64(X, 63);
-            if (l + 1 > n) {
-                if (xmax <= readlane64(L, 0)) break;
-                wave_merge(X, L);
-                store(l, L);
-                break;
-            }
-            if (xmax <= readlane64(L, 0) && xmax <= readlane64(R, 0)) break;
-            uint64_t lmax = readlane64(L, 63), rmax = readlane64(R, 63);
-            int big = (lmax >= rmax) ? l : l + 1;   // larger half replaces the child with the larger max
-            int small = (big == l) ? l + 1 : l;
-            const int nl = 2 * small;                // the next level: children of `small`
-            const uint64_t NL = nl <= n ? load(nl) : SENT;
-            const uint64_t NR = nl + 1 <= n ? load(nl + 1) : SENT;
-            wave_merge(L, R);                        // L = smaller half, R = larger half
-            store(big, R);
-            wave_merge(X, L);                        // X keeps the 64 smallest
-            if (v == 1) root = X;
-            else store(v, X);
-            v = small;
-            X = L;
-            l = nl;
-            L = NL;
-            R = NR;
-        }
-        if (v == 1) root = X;
-        else store(v, X);
-    }
-
-    // Pops the (up to) 64 smallest keys, one per lane, ascending; SENT where empty. The
-    // root's children that remain after the last node moves up are loaded with it.
+    // Pops the (up to) 64 smallest keys, one per lane, ascending; SENT where empty.
+    //
+    // The root becomes a hole that descends: at each level the hole's two children
+    // merge, the low half fills the hole and the high half replaces the child with the
+    // larger max (its subtree's keys are at least that max); the other child is the new
+    // hole. One merge a level, with the next level's children loaded before it. At a
+    // leaf the hole takes the last node, which moves up while its parent's max exceeds
+    // its min (rarely: the last node holds late, large keys). Finally the partial batch
+    // gives up any keys below the new root's max (H2).
     __device__ uint64_t pop() {
         uint64_t out;
         if (n >= 1) {
             out = root;
             if (n >= 2) {
-                const uint64_t L = n >= 3 ? load(2) : SENT;
-                const uint64_t R = n >= 4 ? load(3) : SENT;
-                root = load(n);
-                --n;
-                wave_merge(root, part);  // root takes 64 of (last node U partial), partial keeps pc
-                sift_down(L, R);
+                int v = 1;
+                uint64_t L = load(2);
+                uint64_t R = n >= 3 ? load(3) : SENT;
+                while (true) {
+                    const int l = 2 * v;  // L, R hold nodes l and l + 1 (R = SENT beyond n)
+                    if (l + 1 > n) {      // a single child: it moves up into the hole
+                        if (v == 1) root = L;
+                        else store(v, L);
+                        v = l;
+                        break;
+                    }
+                    const uint64_t lmax = readlane64(L, 63), rmax = readlane64(R, 63);
+                    const int big = lmax >= rmax ? l : l + 1;
+                    const int small = big == l ? l + 1 : l;
+                    const int nl = 2 * small;
+                    const uint64_t NL = nl <= n ? load(nl) : SENT;
+                    const uint64_t NR = nl + 1 <= n ? load(nl + 1) : SENT;
+                    wave_merge(L, R);  // L = low half, R = high half
+                    store(big, R);
+                    if (v == 1) root = L;
+                    else store(v, L);
+                    v = small;
+                    if (nl > n) break;  // the hole is a leaf
+                    L = NL;
+                    R = NR;
+                }
+                if (v != n) {  // fill the hole with the last node and let it rise
+                    uint64_t Y = load(n);
+                    --n;
+                    int u = v;
+                    while (u > 1) {
+                        const int p = u >> 1;
+                        uint64_t P = p == 1 ? root : load(p);
+                        if (readlane64(P, 63) <= readlane64(Y, 0)) break;
+                        wave_merge(P, Y);  // P = low half (rises), Y = high half (stays at u)
+                        store(u, Y);
+                        u = p;
+                        Y = P;
+                    }
+                    if (u == 1) root = Y;
+                    else store(u, Y);
+                } else {
+                    --n;
+                }
+                wave_merge(root, part);  // root keeps the 64 smallest, partial keeps pc keys
             } else {
                 n = 0;
                 root = SENT;
