@@ -91,7 +91,7 @@ __device__ __forceinline__ uint64_t wave_sort(uint64_t v) {
 }
 
 // In: a, b ascending. Out: a = the 64 smallest of a U b, b = the 64 largest, both ascending.
-__device__ __forceinline__ void wave_merge(uint64_t &a, uint64_t &b) {
+__device__ __forceinline__ void wave_merge_net(uint64_t &a, uint64_t &b) {
     const int l = lane_id();
     uint64_t br = shfl64(b, 63 - l);
     uint64_t lo = umin64(a, br), hi = umax64(a, br);
@@ -104,6 +104,18 @@ __device__ __forceinline__ void wave_merge(uint64_t &a, uint64_t &b) {
     }
     a = lo;
     b = hi;
+}
+// The same, skipping the network when the two batches do not overlap (wave-uniform
+// test on two lanes): a <= b already, or b <= a and the batches only trade places.
+__device__ __forceinline__ void wave_merge(uint64_t &a, uint64_t &b) {
+    if (readlane64(a, 63) <= readlane64(b, 0)) return;
+    if (readlane64(b, 63) <= readlane64(a, 0)) {
+        const uint64_t t = a;
+        a = b;
+        b = t;
+        return;
+    }
+    wave_merge_net(a, b);
 }
 
 // Exclusive prefix sum of a per-lane count; returns the wave total in *total.
