@@ -74,34 +74,65 @@ __device__ __forceinline__ uint64_t readlane64(uint64_t v, int lane) {
 __device__ __forceinline__ uint64_t umin64(uint64_t a, uint64_t b) { return a < b ? a : b; }
 __device__ __forceinline__ uint64_t umax64(uint64_t a, uint64_t b) { return a < b ? b : a; }
 
+// One compare-exchange between lanes l and l ^ J (J a power of two below 64): a lane
+// whose want_min is set keeps the smaller key of the pair, the other lane the larger.
+// One 64-bit compare and two selects per stage. For J = 16 and 32 the gfx950 permlane
+// swaps of the key with itself hand every lane both keys of its pair (x from the pair's
+// lower lane, y from its upper lane), so no lane-dependent pick of the partner is needed.
+template <int J>
+__device__ __forceinline__ uint64_t cmpx(uint64_t v, bool want_min) {
+#ifndef UA_EMU
+    if constexpr (J == 16 || J == 32) {
+        const uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+        auto pl = J == 16 ? __builtin_amdgcn_permlane16_swap(lo, lo, false, false)
+                          : __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+        auto ph = J == 16 ? __builtin_amdgcn_permlane16_swap(hi, hi, false, false)
+                          : __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+        const uint64_t x = ((uint64_t)ph[0] << 32) | pl[0];
+        const uint64_t y = ((uint64_t)ph[1] << 32) | pl[1];
+        return ((y < x) == want_min) ? y : x;
+    }
+#endif
+    const uint64_t o = shfl_xor64(v, J);
+    return ((o < v) == want_min) ? o : v;
+}
+
 // Ascending bitonic sort of one key per lane.
 __device__ __forceinline__ uint64_t wave_sort(uint64_t v) {
     const int l = lane_id();
-#pragma unroll
-    for (int k = 2; k <= 64; k <<= 1) {
-#pragma unroll
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            uint64_t o = shfl_xor64(v, j);
-            bool up = (l & k) == 0;
-            bool lower = (l & j) == 0;
-            v = (lower == up) ? umin64(v, o) : umax64(v, o);
-        }
-    }
+#define UA_SORT_STAGE(K, J) v = cmpx<J>(v, ((l & (J)) == 0) == ((l & (K)) == 0))
+    UA_SORT_STAGE(2, 1);
+    UA_SORT_STAGE(4, 2); UA_SORT_STAGE(4, 1);
+    UA_SORT_STAGE(8, 4); UA_SORT_STAGE(8, 2); UA_SORT_STAGE(8, 1);
+    UA_SORT_STAGE(16, 8); UA_SORT_STAGE(16, 4); UA_SORT_STAGE(16, 2); UA_SORT_STAGE(16, 1);
+    UA_SORT_STAGE(32, 16); UA_SORT_STAGE(32, 8); UA_SORT_STAGE(32, 4); UA_SORT_STAGE(32, 2); UA_SORT_STAGE(32, 1);
+    UA_SORT_STAGE(64, 32); UA_SORT_STAGE(64, 16); UA_SORT_STAGE(64, 8); UA_SORT_STAGE(64, 4); UA_SORT_STAGE(64, 2);
+    UA_SORT_STAGE(64, 1);
+#undef UA_SORT_STAGE
     return v;
 }
 
 // In: a, b ascending. Out: a = the 64 smallest of a U b, b = the 64 largest, both ascending.
+// Lane l first compares a[l] with b[63 - l]; the two halves are then bitonic and each
+// is cleaned by six half-cleaner stages.
 __device__ __forceinline__ void wave_merge_net(uint64_t &a, uint64_t &b) {
     const int l = lane_id();
-    uint64_t br = shfl64(b, 63 - l);
-    uint64_t lo = umin64(a, br), hi = umax64(a, br);
-#pragma unroll
-    for (int j = 32; j > 0; j >>= 1) {
-        uint64_t o1 = shfl_xor64(lo, j), o2 = shfl_xor64(hi, j);
-        bool upper = (l & j) != 0;
-        lo = upper ? umax64(lo, o1) : umin64(lo, o1);
-        hi = upper ? umax64(hi, o2) : umin64(hi, o2);
-    }
+    const uint64_t br = shfl64(b, 63 - l);
+    const bool c = br < a;
+    uint64_t lo = c ? br : a, hi = c ? a : br;
+#define UA_MERGE_STAGE(J)                 \
+    do {                                  \
+        const bool wm = (l & (J)) == 0;   \
+        lo = cmpx<J>(lo, wm);             \
+        hi = cmpx<J>(hi, wm);             \
+    } while (0)
+    UA_MERGE_STAGE(32);
+    UA_MERGE_STAGE(16);
+    UA_MERGE_STAGE(8);
+    UA_MERGE_STAGE(4);
+    UA_MERGE_STAGE(2);
+    UA_MERGE_STAGE(1);
+#undef UA_MERGE_STAGE
     a = lo;
     b = hi;
 }
