@@ -100,6 +100,16 @@ inline T __shfl_up(T v, unsigned d, int width = 64) {
     uint64_t r = emu::exchange((uint64_t)(std::make_unsigned_t<T>)v, l >= d ? l - d : l);
     return (T)r;
 }
+// Mask of the lanes whose predicate holds (every lane reads the whole exchange buffer).
+inline uint64_t __ballot(int pred) {
+    emu::Wave &w = emu::block->waves[emu::tid >> 6];
+    w.buf[emu::tid & 63] = pred != 0;
+    w.bar.arrive_and_wait();
+    uint64_t m = 0;
+    for (int i = 0; i < 64; ++i) m |= (w.buf[i] & 1u) << i;
+    w.bar.arrive_and_wait();
+    return m;
+}
 inline int __builtin_amdgcn_readlane(int v, int lane) {
     return (int)(uint32_t)emu::exchange((uint64_t)(uint32_t)v, (unsigned)lane);
 }

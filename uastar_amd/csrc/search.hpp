@@ -740,10 +740,14 @@ __device__ void team_solve(const QueryDesc &q, QueryResult *res, const Team &tea
 #pragma unroll
         for (int i = 0; i < Prob::MAXS; ++i)
             if (s[i] != SENT) *out++ = s[i];
-        goalmin = wave_min64(goalmin);
-        if (l == 0 && goalmin != SENT) sc.fmin(&cur->goal, goalmin);
-        for (int j = 32; j > 0; j >>= 1) err |= __shfl_xor(err, j);
-        if (l == 0 && err) sc.fold_or(&cur->err, err);
+        if (__ballot(goalmin != SENT)) {  // rare: reduce only in rounds that reach a goal
+            goalmin = wave_min64(goalmin);
+            if (l == 0) sc.fmin(&cur->goal, goalmin);
+        }
+        if (__ballot(err != 0)) {
+            for (int j = 32; j > 0; j >>= 1) err |= __shfl_xor(err, j);
+            if (l == 0) sc.fold_or(&cur->err, err);
+        }
 
         RoundSlot *nxt = &slots[(r + 1) & 1];
         TeamVars *const tvp = team.tv;
