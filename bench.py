@@ -1,6 +1,6 @@
 """Benchmark of the GA* solve path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload plain-grid1000]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload plain-grid1000|deep-grid]
     python bench.py --copy-calib 1024     # one 1 GiB device-to-device copy (counter calibration)
 
 Prints ONE JSON line (see the keys below). A "step" is one query of the workload's
@@ -399,6 +399,59 @@ def run_serve_mixed(args):
     return out
 
 
+def run_deep_grid(args):
+    """The serving stream's deepest grid (largest start-goal Chebyshev distance: 7786^2,
+    p = 0.2, 7149 steps apart) solved alone by one one-block team (ua_batch_solve of one
+    query), `steps` times after `warmup` small grids. Its rounds are set by the path's
+    depth, so the time per round is the figure that bounds the serving stream's tail."""
+    import uastar_amd as U
+    ctx = U.Context(0)
+    specs = [s for s in mixed_stream(204) if s["kind"] == "grid"]
+    deep = max(specs, key=lambda s: max(abs(s["s"][0] - s["g"][0]), abs(s["s"][1] - s["g"][1])))
+    warm = [s for s in specs if s["H"] < 1000][: args.warmup]
+    materialize(ctx, warm + [deep])
+    for s in warm:
+        ctx.batch_solve([ctx.grid_query(s["grid"], *s["s"], *s["g"])])
+    q = ctx.grid_query(deep["grid"], *deep["s"], *deep["g"])
+    res, lat = [], []
+    for i in range(args.steps):
+        t = time.perf_counter()
+        res.append(ctx.batch_solve([q])[0])
+        lat.append((time.perf_counter() - t) * 1e3)
+        say(f"deep grid solve {i}: {lat[-1]:.1f} ms, {res[-1]['rounds']} rounds")
+    r = res[-1]
+    dev = [x["device_ms"] for x in res]
+    alg = [grid_bytes_alg(x) for x in res]
+    out = {
+        "metric": "queries/s + p50/p99 latency, batched serving vs plain GA*, 1 MI355X, exact",
+        "value": len(lat) / (sum(lat) / 1e3),
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": len(lat),
+        "warmup": len(warm),
+        "ms_per_step": float(np.mean(lat)),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int64",
+        "data": "synthetic (seeded generators)",
+        "config": {"workload": f"the serving stream's deepest grid alone: {deep['H']}x{deep['W']}, "
+                               f"p = {deep['p']:.3f}, start {tuple(deep['s'])}, goal {tuple(deep['g'])}",
+                   "mode": "batch of one query (one one-block team)"},
+        "errors": sum(x["status"] < 0 for x in res),
+        "found": bool(r["found"]), "orth": r["orth"], "diag": r["diag"],
+        "rounds": r["rounds"], "expanded": r["expanded"], "pops": r["pops"], "inserted": r["inserted"],
+        "device_ms": float(np.mean(dev)), "us_per_round": 1e3 * float(np.mean(dev)) / max(1, r["rounds"]),
+        "roofline": {"bound": "hbm", "achieved": sum(alg) / (sum(dev) * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": sum(alg) / (sum(dev) * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_teams<1, 0> (batch launch)", "kernel_ms_avg": float(np.mean(dev))},
+    }
+    for s in warm + [deep]:
+        s["grid"].free()
+    ctx.close()
+    return out
+
+
 # ------------------------------------------------------------------ main
 
 def main():
@@ -406,7 +459,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=4)
-    ap.add_argument("--workload", default="serve-mixed", choices=["serve-mixed", "plain-grid1000"])
+    ap.add_argument("--workload", default="serve-mixed", choices=["serve-mixed", "plain-grid1000", "deep-grid"])
     ap.add_argument("--load", type=float, default=20.0, help="arrival rate as a multiple of plain capacity")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default settings)")
@@ -429,6 +482,9 @@ def main():
         sys.exit(1)
     if args.workload == "serve-mixed":
         print(json.dumps(run_serve_mixed(args)))
+        return
+    if args.workload == "deep-grid":
+        print(json.dumps(run_deep_grid(args)))
         return
     ctx = U.Context(0, graphs=args.graph)
 
