@@ -137,7 +137,7 @@ struct GroupTeam {
     // generation moves); the workgroup barriers carry the block's other threads. The
     // spin itself reads relaxed.
     template <class Hook>
-    __device__ bool sync(Hook hook) const {
+    __device__ __forceinline__ bool sync(Hook hook) const {
         if (bpg == 1) {
             sc.release();
             __syncthreads();
@@ -186,6 +186,7 @@ struct GroupTeam {
 // Best word per cell: (g_key << 5) | (open << 4) | dir, dir = move into the cell (8 = start).
 struct GridProb {
     static constexpr int MAXS = 8;
+    static constexpr bool kContinue = true;
     QueryDesc q;
     uint64_t cmask, ymask;
     uint64_t goal_id;
@@ -221,10 +222,11 @@ struct GridProb {
     // accesses: the cell's best word; every in-range neighbour's best word; an atomicMin
     // for each open neighbour (or the goal) the move would improve. Coordinates and
     // offsets are 32-bit; a neighbour's word is addressed from the cell's (rows +-W).
+    // *gk_out receives the cell's g key when the cell was expanded.
     template <class SC>
     __device__ __forceinline__ void expand(const SC &sc, uint64_t it, uint64_t (&s)[MAXS], int &cnt,
                                            uint64_t &goalmin, uint32_t &err, int64_t &n_exp, int64_t &n_gen,
-                                           int64_t &n_ins) const {
+                                           int64_t &n_ins, uint64_t *gk_out = nullptr) const {
         const uint64_t cell = it & cmask;
         const int x = (int)(cell >> q.ybits), y = (int)(cell & ymask);
         if (x >= q.H || y >= q.W) {  // never produced by a correct round: fail the query, touch nothing
@@ -233,11 +235,52 @@ struct GridProb {
         }
         uint64_t *const bp = q.best + ((uint64_t)(uint32_t)x * (uint32_t)q.W + (uint32_t)y);
         const uint64_t bw = sc.load_weak(bp);
-        const int gx = x - q.ex, gy = y - q.ey;  // offsets to the goal
         const uint64_t gk = bw >> kGridKeyShift;
-        if (((gk + oct(uabs(gx), uabs(gy))) >> q.qshift) != (it >> q.cbits)) return;  // stale
+        if (((gk + oct(uabs(x - q.ex), uabs(y - q.ey))) >> q.qshift) != (it >> q.cbits)) return;  // stale
         ++n_exp;
         if (!(bw & kOpenBit)) return;  // source-cell mask rule: a blocked cell has no moves
+        if (gk_out) *gk_out = gk;
+        relax(sc, x, y, gk, bp, s, cnt, goalmin, err, n_gen, n_ins);
+    }
+
+    // Continuation inside the round: when the only successors a lane inserted are a child
+    // whose f equals its parent's (same f field of the key) and at most one other, the
+    // lane expands that child at once instead of pushing it. Its g is the one this lane's
+    // atomicMin just wrote (gk + the move's cost); if another lane lowered it since, that
+    // lane pushed the child with the lower g and this expansion only produced dominated
+    // relaxations. The child is never the goal (the goal is not pushed) and is open (only
+    // open cells are pushed). Its successors exclude the parent, so the lane adds at most
+    // 1 + 7 keys to the round's buffer, within the 8 per pop it is sized for.
+    // Returns the child's key with *dir its move, or SENT.
+    __device__ __forceinline__ uint64_t pick_child(uint64_t it, const uint64_t (&s)[MAXS], int cnt, int *dir) const {
+        uint64_t best = SENT;
+        int bd = 0;
+#pragma unroll
+        for (int d = 0; d < MAXS; ++d)
+            if (s[d] < best) {
+                best = s[d];
+                bd = d;
+            }
+        *dir = bd;
+        return (cnt >= 1 && cnt <= 2 && (best >> q.cbits) == (it >> q.cbits)) ? best : SENT;
+    }
+    template <class SC>
+    __device__ __forceinline__ void expand_child(const SC &sc, uint64_t child, uint64_t gk, uint64_t (&s)[MAXS],
+                                                 int &cnt, uint64_t &goalmin, uint32_t &err, int64_t &n_exp,
+                                                 int64_t &n_gen, int64_t &n_ins) const {
+        const uint64_t cell = child & cmask;
+        const int x = (int)(cell >> q.ybits), y = (int)(cell & ymask);
+        uint64_t *const bp = q.best + ((uint64_t)(uint32_t)x * (uint32_t)q.W + (uint32_t)y);
+        ++n_exp;
+        relax(sc, x, y, gk, bp, s, cnt, goalmin, err, n_gen, n_ins);
+    }
+
+    // Relaxes the moves out of open cell (x, y) of g key gk (bp its best word).
+    template <class SC>
+    __device__ __forceinline__ void relax(const SC &sc, int x, int y, uint64_t gk, uint64_t *bp, uint64_t (&s)[MAXS],
+                                          int &cnt, uint64_t &goalmin, uint32_t &err, int64_t &n_gen,
+                                          int64_t &n_ins) const {
+        const int gx = x - q.ex, gy = y - q.ey;  // offsets to the goal
         const int64_t Wl = q.W;
         uint64_t w[8];
         uint32_t live = 0;
@@ -402,6 +445,7 @@ __device__ __forceinline__ int puzzle_h(const PTab &tb, uint64_t s, const uint8_
 // The side n (3 or 4) is a run-time value.
 struct PuzzleProb {
     static constexpr int MAXS = 4;
+    static constexpr bool kContinue = false;
     static constexpr int MAXPROBE = 256;
     QueryDesc q;
     const uint8_t *pdb;
@@ -409,7 +453,7 @@ struct PuzzleProb {
     int n;
     uint64_t hmask;
     uint64_t goal_b;
-    __device__ PuzzleProb(const QueryDesc &qd, const uint8_t *p) : q(qd), pdb(p), tb(kPTab[qd.n == 3 ? 0 : 1]) {
+    __device__ __forceinline__ PuzzleProb(const QueryDesc &qd, const uint8_t *p) : q(qd), pdb(p), tb(kPTab[qd.n == 3 ? 0 : 1]) {
         n = qd.n == 3 ? 3 : 4;
         hmask = (1ull << q.hash_log2) - 1;
         goal_b = goal_board(n);
@@ -545,6 +589,7 @@ __constant__ static const int8_t kPart5[25] = {-1, 1, 1, 0, 0, 0, 1, 2, 2, 2, 0,
 // h is the parent's (from its item) with the moved tile's partition term swapped.
 struct PuzzleProb5 {
     static constexpr int MAXS = 4;
+    static constexpr bool kContinue = false;
     static constexpr int MAXPROBE = 256;
     static constexpr int SPIN = 1 << 16;
     QueryDesc q;
@@ -659,7 +704,7 @@ struct PuzzleProb5 {
 
 // ------------------------------------------------------------------ the team search
 template <class Team, class Prob>
-__device__ void team_solve(const QueryDesc &q, QueryResult *res, const Team &team, const Prob &P) {
+__device__ __forceinline__ void team_solve(const QueryDesc &q, QueryResult *res, const Team &team, const Prob &P) {
     const Scope sc = team.sc;
     const int l = lane_id();
     const int tw = team.tw, W = team.W;
@@ -727,19 +772,44 @@ __device__ void team_solve(const QueryDesc &q, QueryResult *res, const Team &tea
         int c = 0;
         uint64_t goalmin = SENT;
         uint32_t err = 0;
+        uint64_t gk = 0;
         if (it != SENT) {
             ++n_pop;
-            P.expand(sc, it, s, c, goalmin, err, n_exp, n_gen, n_ins);
+            if constexpr (Prob::kContinue) P.expand(sc, it, s, c, goalmin, err, n_exp, n_gen, n_ins, &gk);
+            else P.expand(sc, it, s, c, goalmin, err, n_exp, n_gen, n_ins);
         }
-        int total;
-        int off = wave_excl_scan(c, &total);
-        uint32_t base = 0;
-        if (l == 0 && total) base = sc.fadd(&cur->cnt, (uint32_t)total);
-        base = __shfl(base, 0);
-        uint64_t *out = cur_seg + base + off;
+        // appends the lane's keys -- s[] and `keep` unless SENT, c in all -- to the round's
+        // insert buffer
+        uint64_t keep = SENT;
+        auto append = [&] {
+            int total;
+            const int off = wave_excl_scan(c, &total);
+            uint32_t base = 0;
+            if (l == 0 && total) base = sc.fadd(&cur->cnt, (uint32_t)total);
+            base = __shfl(base, 0);
+            uint64_t *out = cur_seg + base + off;
 #pragma unroll
-        for (int i = 0; i < Prob::MAXS; ++i)
-            if (s[i] != SENT) *out++ = s[i];
+            for (int i = 0; i < Prob::MAXS; ++i)
+                if (s[i] != SENT) *out++ = s[i];
+            if (keep != SENT) *out = keep;
+        };
+        if constexpr (Prob::kContinue) {
+            int dc = 0;
+            const uint64_t child = it != SENT ? P.pick_child(it, s, c, &dc) : SENT;
+            if (__ballot(child != SENT)) {
+                if (child != SENT) {  // the child's sibling, if any, waits in `keep`
+#pragma unroll
+                    for (int i = 0; i < Prob::MAXS; ++i) {
+                        if (i != dc && s[i] < keep) keep = s[i];
+                        s[i] = SENT;
+                    }
+                    c = keep != SENT;
+                    P.expand_child(sc, child, gk + (dc < 4 ? KEY_ONE : KEY_SQRT2), s, c, goalmin, err, n_exp, n_gen,
+                                   n_ins);
+                }
+            }
+        }
+        append();
         if (__ballot(goalmin != SENT)) {  // rare: reduce only in rounds that reach a goal
             goalmin = wave_min64(goalmin);
             if (l == 0) sc.fmin(&cur->goal, goalmin);
