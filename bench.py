@@ -282,7 +282,7 @@ def run_serve_mixed(args):
     tuning = None
     if not args.no_tune:
         # the autotuner measures on its own sample (another seed of the same stream)
-        tspecs = mixed_stream(24, seed=77, max_side=4096)
+        tspecs = mixed_stream(24, seed=77)
         materialize(ctx, tspecs)
         tq = [ctx.grid_query(s["grid"], *s["s"], *s["g"]) if s["kind"] == "grid" else ctx.puzzle_query(4, s["board"])
               for s in tspecs]
