@@ -1,6 +1,6 @@
 """Benchmark of the GA* solve path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload plain-grid1000|deep-grid]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload plain-grid1000|plain-grid16384|deep-grid]
     python bench.py --copy-calib 1024     # one 1 GiB device-to-device copy (counter calibration)
 
 Prints ONE JSON line (see the keys below). A "step" is one query of the workload's
@@ -83,6 +83,11 @@ def pct(xs, q):
 def grid1000_specs(n, seed0=1):
     """1000x1000 grids, obstacles i.i.d. Bernoulli(0.30), start (0,0), goal (999,999)."""
     return [dict(kind="grid", H=1000, W=1000, seed=seed0 + i, p=0.30, s=(0, 0), g=(999, 999)) for i in range(n)]
+
+
+def grid16384_specs(n, seed0=2):
+    """16384x16384 grids (1 bit per cell), obstacles i.i.d. Bernoulli(0.35), opposite corners."""
+    return [dict(kind="grid", H=16384, W=16384, seed=seed0 + i, p=0.35, s=(0, 0), g=(16383, 16383)) for i in range(n)]
 
 
 def run_plain(ctx, specs, warm_specs):
@@ -459,7 +464,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=4)
-    ap.add_argument("--workload", default="serve-mixed", choices=["serve-mixed", "plain-grid1000", "deep-grid"])
+    ap.add_argument("--workload", default="serve-mixed", choices=["serve-mixed", "plain-grid1000", "plain-grid16384", "deep-grid"])
     ap.add_argument("--load", type=float, default=20.0, help="arrival rate as a multiple of plain capacity")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default settings)")
@@ -488,11 +493,13 @@ def main():
         return
     ctx = U.Context(0, graphs=args.graph)
 
-    specs = grid1000_specs(args.steps)
-    warm = grid1000_specs(args.warmup, seed0=1_000_000)
+    big = args.workload == "plain-grid16384"
+    specs = grid16384_specs(args.steps) if big else grid1000_specs(args.steps)
+    warm = grid16384_specs(args.warmup, seed0=1_000_000) if big else grid1000_specs(args.warmup, seed0=1_000_000)
     wall, lat, res = run_plain(ctx, specs, warm)
     qps = len(lat) / wall
-    bad = [r for r in res if r["status"] != 0]
+    bad = [r for r in res if r["status"] < 0]  # errors; a grid with no path is an answer
+    no_path = sum(r["status"] == 1 for r in res)
     alg = [grid_bytes_alg(r) for r in res]
     kms = [r["kernel_ms"] for r in res]
     achieved = sum(alg) / (sum(kms) * 1e-3) / 1e9
@@ -509,12 +516,17 @@ def main():
         "vs_baseline": None,
         "dtype": "int64",
         "data": "synthetic (seeded generators)",
-        "config": {"workload": "plain GA*: 1000x1000 grids, Bernoulli(0.30) obstacles from seeds 1.., "
-                               "start (0,0), goal (999,999), one query per whole-GPU launch",
-                   "mode": "plain", "heaps": 4 * 256},
+        "config": {"workload": ("plain GA*: 16384x16384 grids, Bernoulli(0.35) obstacles from seeds 2.., "
+                                "start (0,0), goal (16383,16383), one query per whole-GPU launch") if big else
+                               ("plain GA*: 1000x1000 grids, Bernoulli(0.30) obstacles from seeds 1.., "
+                                "start (0,0), goal (999,999), one query per whole-GPU launch"),
+                   "mode": "plain", "heaps": ctx.get_tuning()["plain_grid_heaps"]},
+        "rounds_p50": pct([r["rounds"] for r in res], 50),
+        "answers": [(r["orth"], r["diag"]) for r in res[:4]],
         "p50_ms": pct(lat, 50),
         "p99_ms": pct(lat, 99),
         "errors": len(bad),
+        "no_path": no_path,
         "expanded_per_s": sum(r["expanded"] for r in res) / wall,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
@@ -522,7 +534,7 @@ def main():
                      "alg_bytes_avg": float(np.mean(alg))},
     }
     say(f"GPU: {qps:.2f} queries/s, p50 {out['p50_ms']:.2f} ms, {len(bad)} errors")
-    if not args.no_cpu:
+    if not args.no_cpu and not big:
         say("CPU baseline (oracle, bounded sample)")
         out["cpu_baseline"] = cpu_baseline_grid(specs)
     ctx.close()

@@ -794,8 +794,12 @@ __device__ __forceinline__ void team_solve(const QueryDesc &q, QueryResult *res,
             if (keep != SENT) *out = keep;
         };
         if constexpr (Prob::kContinue) {
+            // Only a wave whose heap still holds keys after its pop continues: its pops were
+            // the smallest of more keys than a round takes, so best-first order still holds.
+            // When the open list fits in one round's pops, continuation would expand children
+            // on g values that later rounds correct, and their re-expansions multiply.
             int dc = 0;
-            const uint64_t child = it != SENT ? P.pick_child(it, s, c, &dc) : SENT;
+            const uint64_t child = it != SENT && heap.size() > 0 ? P.pick_child(it, s, c, &dc) : SENT;
             if (__ballot(child != SENT)) {
                 if (child != SENT) {  // the child's sibling, if any, waits in `keep`
 #pragma unroll

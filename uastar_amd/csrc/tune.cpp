@@ -156,7 +156,7 @@ extern "C" int ua_autotune(ua_ctx *c, const ua_query *qs, int64_t n, ua_tuning *
         for (int kind : {KIND_GRID, KIND_PUZZLE}) {
             const std::vector<ua_query> &src = kind == KIND_GRID ? grids : puzzles;
             if (src.empty()) continue;
-            const size_t m = std::min<size_t>(3, src.size());
+            const size_t m = std::min<size_t>(16, src.size());
             int best_h = 0;
             double best = 0, best_ins = 0;
             // depth-bound grids can run best on one block (fewer, fuller heaps); frontier-
