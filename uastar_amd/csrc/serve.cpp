@@ -164,14 +164,20 @@ int plan_job(ua_ctx *c, ServeEngine *e, Job &j, bool *fits) {
     j.d.resume = reinterpret_cast<uint64_t *>(base + o);
     // fresh search state: grid best = all ones but blocked cells' open bit; puzzle keys = 0,
     // vals = all ones; no resume record
+    hipError_t he;
     if (j.d.kind == KIND_GRID) {
-        HIPCHK(c, hipMemsetAsync(j.d.best, 0xFF, j.plan.best_bytes, c->st));
-        HIPCHK(c, launch_grid_mark_blocked(j.d.best, j.d.bitmap, j.d.H, j.d.W, j.d.wpr, c->st));
+        he = hipMemsetAsync(j.d.best, 0xFF, j.plan.best_bytes, c->st);
+        if (he == hipSuccess) he = launch_grid_mark_blocked(j.d.best, j.d.bitmap, j.d.H, j.d.W, j.d.wpr, c->st);
     } else {
-        HIPCHK(c, hipMemsetAsync(j.d.best, 0, j.plan.best_bytes, c->st));
-        HIPCHK(c, hipMemsetAsync(j.d.vals, 0xFF, j.plan.vals_bytes, c->st));
+        he = hipMemsetAsync(j.d.best, 0, j.plan.best_bytes, c->st);
+        if (he == hipSuccess) he = hipMemsetAsync(j.d.vals, 0xFF, j.plan.vals_bytes, c->st);
     }
-    HIPCHK(c, hipMemsetAsync(j.d.resume, 0, 64, c->st));
+    if (he == hipSuccess) he = hipMemsetAsync(j.d.resume, 0, 64, c->st);
+    if (he != hipSuccess) {  // the job is not admitted: give its region back
+        e->pool.release(off, j.bytes);
+        *fits = false;
+        HIPCHK(c, he);
+    }
     return UA_OK;
 }
 
@@ -189,13 +195,19 @@ int widen(ua_ctx *c, ServeEngine *e, Job &j, int waves) {
     char *base2 = e->pool.arena.as<char>() + off2;
     uint64_t *heap2 = reinterpret_cast<uint64_t *>(base2);
     uint64_t *rec2 = reinterpret_cast<uint64_t *>(base2 + align256(heap_bytes));
-    HIPCHK(c, hipMemcpyAsync(heap2, j.d.heap, (size_t)w0 * per_heap, hipMemcpyDeviceToDevice, c->st));
-    HIPCHK(c, hipMemcpyAsync(rec2, j.d.resume, (size_t)resume_words(w0) * 8, hipMemcpyDeviceToDevice, c->st));
     // the new heaps: empty root and partial batches (all ones), n = pc = 0
     std::vector<uint64_t> fresh((size_t)(waves - w0) * RESUME_PER_HEAP, ~0ull);
     for (int h = 0; h < waves - w0; ++h) fresh[(size_t)h * RESUME_PER_HEAP + 128] = fresh[(size_t)h * RESUME_PER_HEAP + 129] = 0;
-    HIPCHK(c, hipMemcpyAsync(rec2 + resume_words(w0), fresh.data(), fresh.size() * 8, hipMemcpyHostToDevice, c->st));
-    HIPCHK(c, hipStreamSynchronize(c->st));
+    hipError_t he = hipMemcpyAsync(heap2, j.d.heap, (size_t)w0 * per_heap, hipMemcpyDeviceToDevice, c->st);
+    if (he == hipSuccess)
+        he = hipMemcpyAsync(rec2, j.d.resume, (size_t)resume_words(w0) * 8, hipMemcpyDeviceToDevice, c->st);
+    if (he == hipSuccess)
+        he = hipMemcpyAsync(rec2 + resume_words(w0), fresh.data(), fresh.size() * 8, hipMemcpyHostToDevice, c->st);
+    if (he == hipSuccess) he = hipStreamSynchronize(c->st);
+    if (he != hipSuccess) {  // the query keeps its narrow team; give the wide region back
+        e->pool.release(off2, bytes2);
+        HIPCHK(c, he);
+    }
     const size_t keep = align256(j.plan.best_bytes) + align256(j.plan.vals_bytes);
     e->pool.release(j.off + keep, j.bytes - keep);  // the narrow team's heaps and record
     j.bytes = keep;
