@@ -91,6 +91,34 @@ struct Scope {
                   : __hip_atomic_compare_exchange_strong(p, &expect, v, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
                                                          __HIP_MEMORY_SCOPE_AGENT);
     }
+#ifndef UA_EMU
+    // The same operations on pointers known to be in device memory (global instructions).
+    template <class T>
+    __device__ __forceinline__ T load(const UA_GLOBAL T *p) const {
+        return wg ? __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
+                  : __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    template <class T>
+    __device__ __forceinline__ T load_weak(const UA_GLOBAL T *p) const {
+        return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    template <class T>
+    __device__ __forceinline__ void store(UA_GLOBAL T *p, T v) const {
+        if (wg) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        else __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    template <class T>
+    __device__ __forceinline__ T fmin(UA_GLOBAL T *p, T v) const {
+        return wg ? __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
+                  : __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __device__ __forceinline__ bool cas(UA_GLOBAL uint64_t *p, uint64_t &expect, uint64_t v) const {
+        return wg ? __hip_atomic_compare_exchange_strong(p, &expect, v, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_WORKGROUP)
+                  : __hip_atomic_compare_exchange_strong(p, &expect, v, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_AGENT);
+    }
+#endif
     __device__ __forceinline__ void release() const {
         if (wg) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         else __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
@@ -233,7 +261,7 @@ struct GridProb {
             err |= FLAG_ABORT;
             return;
         }
-        uint64_t *const bp = q.best + ((uint64_t)(uint32_t)x * (uint32_t)q.W + (uint32_t)y);
+        UA_GLOBAL uint64_t *const bp = gptr(q.best) + ((uint64_t)(uint32_t)x * (uint32_t)q.W + (uint32_t)y);
         const uint64_t bw = sc.load_weak(bp);
         const uint64_t gk = bw >> kGridKeyShift;
         if (((gk + oct(uabs(x - q.ex), uabs(y - q.ey))) >> q.qshift) != (it >> q.cbits)) return;  // stale
@@ -270,14 +298,15 @@ struct GridProb {
                                                  int64_t &n_gen, int64_t &n_ins) const {
         const uint64_t cell = child & cmask;
         const int x = (int)(cell >> q.ybits), y = (int)(cell & ymask);
-        uint64_t *const bp = q.best + ((uint64_t)(uint32_t)x * (uint32_t)q.W + (uint32_t)y);
+        UA_GLOBAL uint64_t *const bp = gptr(q.best) + ((uint64_t)(uint32_t)x * (uint32_t)q.W + (uint32_t)y);
         ++n_exp;
         relax(sc, x, y, gk, bp, s, cnt, goalmin, err, n_gen, n_ins);
     }
 
     // Relaxes the moves out of open cell (x, y) of g key gk (bp its best word).
     template <class SC>
-    __device__ __forceinline__ void relax(const SC &sc, int x, int y, uint64_t gk, uint64_t *bp, uint64_t (&s)[MAXS],
+    __device__ __forceinline__ void relax(const SC &sc, int x, int y, uint64_t gk, UA_GLOBAL uint64_t *bp,
+                                          uint64_t (&s)[MAXS],
                                           int &cnt, uint64_t &goalmin, uint32_t &err, int64_t &n_gen,
                                           int64_t &n_ins) const {
         const int gx = x - q.ex, gy = y - q.ey;  // offsets to the goal
@@ -436,7 +465,7 @@ __device__ __forceinline__ int puzzle_h(const PTab &tb, uint64_t s, const uint8_
     int h = 0;
 #pragma unroll
     for (int p = 0; p < 2; ++p)
-        if (p < tb.np) h += pdb[tb.off[p] + pattern_rank_rt(tb, s, p)];
+        if (p < tb.np) h += gptr(pdb)[tb.off[p] + pattern_rank_rt(tb, s, p)];
     return h;
 }
 
@@ -463,11 +492,11 @@ struct PuzzleProb {
     __device__ __forceinline__ uint64_t find_or_insert(const SC &sc, uint64_t b, uint32_t &err) const {
         uint64_t slot = mix64(b) & hmask;
         for (int i = 0; i < MAXPROBE; ++i) {
-            uint64_t k = sc.load_weak(&q.best[slot]);
+            uint64_t k = sc.load_weak(gptr(q.best) + slot);
             if (k == b) return slot;
             if (k == 0) {
                 uint64_t exp = 0;
-                if (sc.cas(&q.best[slot], exp, b) || exp == b) return slot;
+                if (sc.cas(gptr(q.best) + slot, exp, b) || exp == b) return slot;
             }
             slot = (slot + 1) & hmask;
         }
@@ -495,8 +524,8 @@ struct PuzzleProb {
             return;
         }
         uint32_t gi = 255u - (uint32_t)((it >> 48) & 255u);
-        uint64_t b = sc.load_weak(&q.best[slot]);
-        uint32_t v = sc.load_weak(&q.vals[slot]);
+        uint64_t b = sc.load_weak(gptr(q.best) + slot);
+        uint32_t v = sc.load_weak(gptr(q.vals) + slot);
         uint32_t gb = v >> 4;
         if (gb != gi) return;  // stale
         ++n_exp;
@@ -521,7 +550,7 @@ struct PuzzleProb {
             }
             uint64_t ns_slot = find_or_insert(sc, ns, err);
             if (ns_slot == SENT) continue;
-            uint32_t old = sc.fmin(&q.vals[ns_slot], (ng << 4) | (uint32_t)d);
+            uint32_t old = sc.fmin(gptr(q.vals) + ns_slot, (ng << 4) | (uint32_t)d);
             if ((old >> 4) <= ng) continue;
             ++n_ins;
             if (ns == goal_b) {

@@ -18,6 +18,20 @@
 
 namespace ua {
 
+// Pointers into device memory (HBM) as the global address space: their loads, stores and
+// atomics compile to global_* instructions (SGPR base plus a lane offset where the base is
+// uniform) instead of flat ones, which also count against the LDS counter, so that a wait
+// for an LDS result does not wait for HBM loads still in flight.
+#ifdef UA_EMU
+#define UA_GLOBAL
+#else
+#define UA_GLOBAL __attribute__((address_space(1)))
+#endif
+template <class T>
+__device__ __forceinline__ UA_GLOBAL T *gptr(T *p) {
+    return (UA_GLOBAL T *)p;
+}
+
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
 __device__ __forceinline__ uint64_t shfl64(uint64_t v, int src) {
@@ -191,10 +205,10 @@ struct WaveHeap {
         nodes = storage;
     }
     __device__ __forceinline__ uint64_t load(int i) const {
-        return nodes[(size_t)(i - 2) * 64 + lane_id()];
+        return gptr(nodes)[(size_t)(i - 2) * 64 + lane_id()];
     }
     __device__ __forceinline__ void store(int i, uint64_t v) const {
-        nodes[(size_t)(i - 2) * 64 + lane_id()] = v;
+        gptr(nodes)[(size_t)(i - 2) * 64 + lane_id()] = v;
     }
     __device__ __forceinline__ uint64_t min_key() const {
         return n >= 1 ? readlane64(root, 0) : readlane64(part, 0);
