@@ -322,30 +322,44 @@ struct GridProb {
         }
         // legal moves: open targets, and the goal even if blocked (any other blocked cell
         // is a dead end); relax with an atomic only where the new cost is lower than what
-        // was read (best words only ever decrease, so a skipped atomic could not succeed)
+        // was read (best words only ever decrease, so a skipped atomic could not succeed).
+        // The atomics go out four at a time (orthogonal moves, then diagonal ones), each
+        // group issued before any of its results is used, so that they are in flight
+        // together; old[j] = 0 marks a move without an atomic.
 #pragma unroll
-        for (int d = 0; d < 8; ++d) {
-            if (!((live >> d) & 1u)) continue;
-            const bool is_goal = gx + kDX[d] == 0 && gy + kDY[d] == 0;
-            if (!is_goal && !(w[d] & kOpenBit)) continue;
-            ++n_gen;
-            const uint64_t ng = gk + (d < 4 ? KEY_ONE : KEY_SQRT2);
-            const uint64_t nf = ng + oct(uabs(gx + kDX[d]), uabs(gy + kDY[d]));
-            if (nf >= KEY_LIMIT) {
-                err |= FLAG_RANGE;
-                continue;
+        for (int d0 = 0; d0 < 8; d0 += 4) {
+            uint64_t old[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int d = d0 + j;
+                old[j] = 0;
+                if (!((live >> d) & 1u)) continue;
+                const bool is_goal = gx + kDX[d] == 0 && gy + kDY[d] == 0;
+                if (!is_goal && !(w[d] & kOpenBit)) continue;
+                ++n_gen;
+                const uint64_t ng = gk + (d < 4 ? KEY_ONE : KEY_SQRT2);
+                const uint64_t nf = ng + oct(uabs(gx + kDX[d]), uabs(gy + kDY[d]));
+                if (nf >= KEY_LIMIT) {
+                    err |= FLAG_RANGE;
+                    continue;
+                }
+                if ((w[d] >> kGridKeyShift) <= ng) continue;
+                old[j] = sc.fmin(bp + (kDX[d] * Wl + kDY[d]), (ng << kGridKeyShift) | (w[d] & kOpenBit) | (uint64_t)d);
+                if (!is_goal)
+                    s[d] = ((nf >> q.qshift) << q.cbits) | ((uint64_t)(uint32_t)(x + kDX[d]) << q.ybits) |
+                           (uint64_t)(uint32_t)(y + kDY[d]);
             }
-            if ((w[d] >> kGridKeyShift) <= ng) continue;
-            const uint64_t old =
-                sc.fmin(bp + (kDX[d] * Wl + kDY[d]), (ng << kGridKeyShift) | (w[d] & kOpenBit) | (uint64_t)d);
-            if ((old >> kGridKeyShift) <= ng) continue;
-            ++n_ins;
-            if (is_goal) {
-                goalmin = goalmin < ng ? goalmin : ng;
-            } else {
-                s[d] = ((nf >> q.qshift) << q.cbits) | ((uint64_t)(uint32_t)(x + kDX[d]) << q.ybits) |
-                       (uint64_t)(uint32_t)(y + kDY[d]);
-                ++cnt;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int d = d0 + j;
+                const uint64_t ng = gk + (d < 4 ? KEY_ONE : KEY_SQRT2);
+                if ((old[j] >> kGridKeyShift) <= ng) {  // no atomic, or it did not improve the cell
+                    s[d] = SENT;
+                    continue;
+                }
+                ++n_ins;
+                if (gx + kDX[d] == 0 && gy + kDY[d] == 0) goalmin = goalmin < ng ? goalmin : ng;
+                else ++cnt;
             }
         }
     }
