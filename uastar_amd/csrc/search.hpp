@@ -543,35 +543,58 @@ struct PuzzleProb {
         uint32_t gb = v >> 4;
         if (gb != gi) return;  // stale
         ++n_exp;
-        int dprev = (int)(v & 15u);
-        int bl = nib_pos(b, 0);
-        int bx = n == 4 ? bl >> 2 : bl / 3, by = bl - bx * n;
+        const int dprev = (int)(v & 15u);
+        const int bl = nib_pos(b, 0);
+        const int bx = n == 4 ? bl >> 2 : bl / 3, by = bl - bx * n;
+        const uint32_t ng = gb + 1;
+        // The moves are handled loop by loop, so that the PDB gathers of all children are in
+        // flight together, and so are the atomicMins on their value words; the hash probes
+        // (a load, then a CAS on an empty slot) stay one child at a time.
+        uint64_t ns[4];
+        int h[4];
+        uint32_t legal = 0;
 #pragma unroll
         for (int d = 0; d < 4; ++d) {
+            ns[d] = 0;
+            h[d] = 0;
             if (dprev < 4 && d == (dprev ^ 1)) continue;  // the move back to the parent
-            int nx = bx + kDX[d], ny = by + kDY[d];
+            const int nx = bx + kDX[d], ny = by + kDY[d];
             if (nx < 0 || nx >= n || ny < 0 || ny >= n) continue;
+            legal |= 1u << d;
+            const int nb = nx * n + ny;
+            const uint64_t t = (b >> (4 * nb)) & 15ull;
+            ns[d] = b + (t << (4 * bl)) - (t << (4 * nb));
+            h[d] = puzzle_h(tb, ns[d], pdb);
+        }
+        uint64_t ts[4];
+#pragma unroll
+        for (int d = 0; d < 4; ++d) {
+            ts[d] = SENT;
+            if (!((legal >> d) & 1u)) continue;
             ++n_gen;
-            int nb = nx * n + ny;
-            uint64_t t = (b >> (4 * nb)) & 15ull;
-            uint64_t ns = b + (t << (4 * bl)) - (t << (4 * nb));
-            uint32_t ng = gb + 1;
-            int h = puzzle_h(tb, ns, pdb);
-            const uint32_t f = (ng >> q.wshift) + (uint32_t)h;
+            const uint32_t f = (ng >> q.wshift) + (uint32_t)h[d];
             if (f > 255u) {
                 err |= q.bounded ? FLAG_PRUNED : FLAG_RANGE;
                 continue;
             }
-            uint64_t ns_slot = find_or_insert(sc, ns, err);
-            if (ns_slot == SENT) continue;
-            uint32_t old = sc.fmin(gptr(q.vals) + ns_slot, (ng << 4) | (uint32_t)d);
-            if ((old >> 4) <= ng) continue;
+            ts[d] = find_or_insert(sc, ns[d], err);
+        }
+        uint32_t old[4];
+#pragma unroll
+        for (int d = 0; d < 4; ++d) {
+            old[d] = 0;  // no atomic: never an improvement
+            if (ts[d] != SENT) old[d] = sc.fmin(gptr(q.vals) + ts[d], (ng << 4) | (uint32_t)d);
+        }
+#pragma unroll
+        for (int d = 0; d < 4; ++d) {
+            if (ts[d] == SENT || (old[d] >> 4) <= ng) continue;
             ++n_ins;
-            if (ns == goal_b) {
-                uint64_t gk = ((uint64_t)ng << 32) | ns_slot;
+            if (ns[d] == goal_b) {
+                const uint64_t gk = ((uint64_t)ng << 32) | ts[d];
                 goalmin = goalmin < gk ? goalmin : gk;
             } else {
-                s[d] = ((uint64_t)f << 56) | ((uint64_t)(255u - ng) << 48) | ns_slot;
+                const uint32_t f = (ng >> q.wshift) + (uint32_t)h[d];
+                s[d] = ((uint64_t)f << 56) | ((uint64_t)(255u - ng) << 48) | ts[d];
                 ++cnt;
             }
         }
