@@ -43,7 +43,8 @@ typedef struct ua_grid ua_grid; /* a device-resident grid bitmap */
 
 typedef struct {
     int32_t team_waves;        /* waves (= heaps) per grid query in batch/serve (multiple of 16); 0 = tuned */
-    int32_t plain_heaps;       /* heaps (= waves) of the plain whole-GPU solve; 0 = tuned (4 per CU) */
+    int32_t plain_heaps;       /* heaps (= waves) of the plain whole-GPU solve; 0 = tuned (untuned: 256 for
+                                  grids, 4 per CU for puzzles) */
     int32_t max_rounds;        /* 0 = 1<<26 */
     int32_t hash_log2;         /* puzzle hash table size per query; 0 = tuned */
     uint64_t mem_budget;       /* device bytes the batch/serve planner may use; 0 = 80-85% of free */

@@ -196,7 +196,11 @@ int plain_heaps(ua_ctx *c, int kind) {
     int per_cu = 0;
     if (teams_max_blocks(&per_cu) != hipSuccess || per_cu < 1) per_cu = 1;
     int tuned = c->tune.plain_heaps[kind == KIND_PUZZLE ? 1 : 0];
-    int want = c->cfg.plain_heaps > 0 ? c->cfg.plain_heaps : (tuned > 0 ? tuned : 4 * c->num_cu);
+    // untuned defaults: 256 heaps (16 blocks) for a grid, whose search is bound by the
+    // depth of its path (1000^2 and 16384^2 grids solve fastest at 128-256 heaps, against
+    // twice the time at 1024); 4 per CU for a puzzle, whose open list is wide
+    const int dflt = kind == KIND_PUZZLE ? 4 * c->num_cu : std::min(256, 4 * c->num_cu);
+    int want = c->cfg.plain_heaps > 0 ? c->cfg.plain_heaps : (tuned > 0 ? tuned : dflt);
     int blocks = std::max(1, (want + kBlockWaves - 1) / kBlockWaves);
     blocks = std::min(blocks, per_cu * c->num_cu);
     return blocks * kBlockWaves;
