@@ -5,7 +5,9 @@
 //   phase A  every wave pops its 64 smallest keys, expands them, relaxes the
 //            successors with one 64-bit atomicMin each (grid: dense best-cost
 //            table; puzzle: lock-free open-addressing hash), and appends the
-//            improved successors to the team's insert buffer;
+//            improved successors to the team's insert buffer. On grids a lane whose
+//            improvements are one equal-f child and at most one other expands that
+//            child in the same round (GridProb::pick_child);
 //   barrier  (the last arriver resets the next round's slot)
 //   phase B  every wave reads the round's minimum popped key (LB) and best goal;
 //            the team stops when goal <= LB; otherwise each wave takes a strided
@@ -16,9 +18,10 @@
 // on the device every round and with duplicate detection done by atomics instead
 // of a sort: an improvement is pushed exactly when the atomicMin lowered the cost.
 //
-// Teams: a GroupTeam is a group of blocks of one cooperative launch. The plain
-// one-query-per-launch solve uses one group of every block; batched and served
-// queries use one block (4 waves, 4 heaps) per query, pulled from a work counter.
+// Teams: a GroupTeam is a group of blocks (16 waves each) of one cooperative launch.
+// The plain one-query-per-launch solve uses one group of plain_heaps / 16 blocks;
+// batched and served queries use a team of their class's width (one block by default
+// for grids, eight for 15-puzzles) per query, pulled from a work counter.
 #pragma once
 #ifndef UA_EMU
 #include <hip/hip_runtime.h>
