@@ -110,17 +110,19 @@ def test_serve_same_alone_and_in_mix(sctx, oracle):
 
 
 def test_serve_widens_long_puzzle(sctx):
-    """A heavy 15-puzzle moves to a wide team partway through and still returns the
-    CPU A* move count; light queries served beside it are unaffected."""
+    """Two heavy 15-puzzles move to wide teams partway through (both at once, the most
+    the serving loop widens) and still return the CPU A* move counts; light queries
+    served beside them are unaffected."""
     ctx = sctx
     rows = sorted(json.load(open(os.path.join(GOLDEN, "c3_puzzles.json"))), key=lambda r: -r["expanded"])
-    heavy = rows[0]
+    heavy = rows[:2]
     light = [r for r in rows if r["expanded"] < 100_000][:4]
     w0 = ctx.serve_stats()["widened"]
-    tickets = {ctx.serve_submit(ctx.puzzle_query(4, heavy["board"])): heavy["steps"]}
+    tickets = {ctx.serve_submit(ctx.puzzle_query(4, r["board"])): r["steps"] for r in heavy}
     for r in light:
         tickets[ctx.serve_submit(ctx.puzzle_query(4, r["board"]))] = r["steps"]
-    _say(f"heavy puzzle seed {heavy['seed']} ({heavy['expanded']} CPU expansions) + {len(light)} light")
+    _say(f"heavy puzzles seeds {[r['seed'] for r in heavy]} ({[r['expanded'] for r in heavy]} CPU expansions) "
+         f"+ {len(light)} light")
     got = _drain(ctx, len(tickets))
     for t, steps in tickets.items():
         assert got[t]["found"] and got[t]["steps"] == steps, (t, got[t], steps)
