@@ -673,25 +673,25 @@ struct PuzzleProb5 {
     __device__ __forceinline__ int h_full(uint64_t w0, uint64_t w1) const {
         int h = 0;
 #pragma unroll
-        for (int p = 0; p < 4; ++p) h += q.pdb[PSpec<5>::OFF[p] + rank5(w0, w1, p)];
+        for (int p = 0; p < 4; ++p) h += gptr(q.pdb)[PSpec<5>::OFF[p] + rank5(w0, w1, p)];
         return h;
     }
     template <class SC>
     __device__ __forceinline__ uint64_t find_or_insert(const SC &sc, uint64_t b0, uint64_t b1, uint32_t &err) const {
         uint64_t slot = mix64(b0 ^ mix64(b1)) & hmask;
         for (int i = 0; i < MAXPROBE; ++i) {
-            uint64_t k0 = sc.load_weak(&q.best[2 * slot]);
+            uint64_t k0 = sc.load_weak(gptr(q.best) + (2 * slot));
             if (k0 == 0) {
                 uint64_t exp = 0;
-                if (sc.cas(&q.best[2 * slot], exp, b0)) {
-                    sc.store(&q.best[2 * slot + 1], b1);
+                if (sc.cas(gptr(q.best) + (2 * slot), exp, b0)) {
+                    sc.store(gptr(q.best) + (2 * slot + 1), b1);
                     return slot;
                 }
                 k0 = exp;
             }
             if (k0 == b0) {
-                uint64_t k1 = sc.load(&q.best[2 * slot + 1]);
-                for (int w = 0; k1 == 0 && w < SPIN; ++w) k1 = sc.load(&q.best[2 * slot + 1]);
+                uint64_t k1 = sc.load(gptr(q.best) + (2 * slot + 1));
+                for (int w = 0; k1 == 0 && w < SPIN; ++w) k1 = sc.load(gptr(q.best) + (2 * slot + 1));
                 if (k1 == 0) {
                     err |= FLAG_ABORT;
                     return SENT;
@@ -707,7 +707,7 @@ struct PuzzleProb5 {
         uint32_t err = 0;
         uint64_t slot = find_or_insert(sc, q.start_state, q.start_state2, err);
         if (slot == SENT) return SENT;
-        sc.store(&q.vals[slot], (uint32_t)15);
+        sc.store(gptr(q.vals) + (slot), (uint32_t)15);
         uint64_t h = (uint64_t)h_full(q.start_state, q.start_state2);
         return (h << 56) | (255ull << 48) | (h << 40) | slot;
     }
@@ -726,9 +726,9 @@ struct PuzzleProb5 {
         }
         uint32_t gi = 255u - (uint32_t)((it >> 48) & 255u);
         int hp = (int)((it >> 40) & 255u);
-        uint64_t w0 = sc.load_weak(&q.best[2 * slot]);
-        uint64_t w1 = sc.load_weak(&q.best[2 * slot + 1]);
-        uint32_t v = sc.load_weak(&q.vals[slot]);
+        uint64_t w0 = sc.load_weak(gptr(q.best) + (2 * slot));
+        uint64_t w1 = sc.load_weak(gptr(q.best) + (2 * slot + 1));
+        uint32_t v = sc.load_weak(gptr(q.vals) + (slot));
         uint32_t gb = v >> 4;
         if (gb != gi) return;  // stale
         ++n_exp;
@@ -748,7 +748,7 @@ struct PuzzleProb5 {
             xor5(n0, n1, nb, t);
             const int p = kPart5[t];
             const uint64_t off = (uint64_t)p * 127512000ull;  // PSpec<5>::OFF[p]
-            int h = hp - (int)q.pdb[off + rank5(w0, w1, p)] + (int)q.pdb[off + rank5(n0, n1, p)];
+            int h = hp - (int)gptr(q.pdb)[off + rank5(w0, w1, p)] + (int)gptr(q.pdb)[off + rank5(n0, n1, p)];
             uint32_t ng = gb + 1;
             const uint32_t f = (ng >> q.wshift) + (uint32_t)h;
             if (f > 255u) {
@@ -757,7 +757,7 @@ struct PuzzleProb5 {
             }
             uint64_t ns_slot = find_or_insert(sc, n0, n1, err);
             if (ns_slot == SENT) continue;
-            uint32_t old = sc.fmin(&q.vals[ns_slot], (ng << 4) | (uint32_t)d);
+            uint32_t old = sc.fmin(gptr(q.vals) + (ns_slot), (ng << 4) | (uint32_t)d);
             if ((old >> 4) <= ng) continue;
             ++n_ins;
             if (n0 == g0 && n1 == g1) {
