@@ -3,11 +3,13 @@
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload plain-grid1000|plain-grid16384|deep-grid]
     python bench.py --copy-calib 1024     # one 1 GiB device-to-device copy (counter calibration)
 
-Prints ONE JSON line (see the keys below). A "step" is one query of the workload's
-stream; `value` is whole-job queries/s with every input already resident in HBM when
-the timed region starts. The dominant kernel's roofline is computed from the
-algorithmic bytes of each query (per-query counters x the per-operation bytes below)
-over the kernel's duration measured with HIP events on the library's stream.
+Prints ONE JSON line (see the keys below). On the serving stream (the default workload)
+a "step" is one batch of QUERIES_PER_STEP arrivals, so `--steps K` serves a stream of
+K * QUERIES_PER_STEP queries; on the plain workloads a step is one query. `value` is
+whole-job queries/s with every input already resident in HBM when the timed region
+starts. The dominant kernel's roofline is computed from the algorithmic bytes of each
+query (per-query counters x the per-operation bytes below) over the kernel's duration
+measured with HIP events on the library's stream.
 
 Algorithmic bytes (per operation, counted by the kernels):
   grid:   16 * pops + 18 * expanded + 8 * generated + 25 * inserted
@@ -29,20 +31,31 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+QUERIES_PER_STEP = 10
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_teams_serve_traffic.md")
 
 
-def measured_traffic():
+def measured_traffic(bench_args):
     """HBM bytes per serving launch from the committed rocprofv3 counter summary
-    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), or None."""
+    (FETCH_SIZE + WRITE_SIZE per dispatch, scaled by the copy-kernel calibration), taken
+    only when that summary profiled this same bench invocation (its `bench_args:` line);
+    otherwise None. Returns (bytes per launch or None, what the figure comes from)."""
+    vals = {}
     try:
         with open(TRAFFIC_PROFILE) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_launch:"):
-                    return float(line.split(":", 1)[1])
+                if ":" in line and not line.startswith((" ", "|", "#", "`")):
+                    k, v = line.split(":", 1)
+                    vals[k.strip()] = v.strip()
     except OSError:
-        pass
-    return None
+        return None, "no counter profile"
+    if vals.get("bench_args") != bench_args:
+        return None, f"counter profile is of bench.py {vals.get('bench_args')!r}, not of this run"
+    try:
+        return float(vals["traffic_bytes_per_launch"]), f"rocprofv3 counters of bench.py {bench_args} " \
+            f"({os.path.relpath(TRAFFIC_PROFILE, ROOT)})"
+    except (KeyError, ValueError):
+        return None, "counter profile without traffic_bytes_per_launch"
 
 
 def copy_calib(mib):
@@ -137,7 +150,9 @@ def cpu_baseline_grid(specs, budget_s=20.0, threads=None):
     with cf.ThreadPoolExecutor(threads) as ex:
         list(ex.map(solve, range(n)))
     wall = time.perf_counter() - t
-    return {"value": n / wall, "unit": "queries/s", "cores": threads, "kind": "port",
+    model, nproc = host_cpu()
+    return {"value": n / wall, "unit": "queries/s", "cores": threads, "kind": "port", "nproc": nproc,
+            "cpu_model": model,
             "sample": f"{n} queries of the same stream on {threads} threads ({wall:.1f} s); "
                       f"one query alone on 1 thread: {one * 1e3:.0f} ms (oracle FP32 variant, reference containers)"}
 
@@ -243,46 +258,98 @@ def serve_pass(ctx, specs, arrivals_ms):
     return res, lat, span, step_wall * 1e3
 
 
-def cpu_baseline_mixed(specs, budget_s=25.0, threads=None):
-    """The oracle (restatement of the reference CPU A*; FP32 grid costs, reference
-    containers) on a bounded sample of the stream, one query per host thread."""
-    from oracle import oracle as O
-    threads = threads or min(16, os.cpu_count() or 1)
-    pdb = None
-    # cap the per-query CPU cost so the sample stays bounded
-    cand = [s for s in specs if (s["kind"] == "grid" and s["H"] <= 2048) or
-            (s["kind"] == "puzzle" and s["expanded"] <= 1_500_000)]
-    sample = cand[: 3 * threads]
+def host_cpu():
+    """The host's CPU model and logical CPU count (as /proc/cpuinfo and os.cpu_count see them)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count()
 
-    def solve(s):
-        nonlocal pdb
+
+def cpu_baseline_mixed(ctx, specs, budget_s=25.0, threads=None, seed=11, puzzle_cap=4_000_000):
+    """The oracle (restatement of the reference CPU A*: FP32 grid costs and the reference's
+    containers; puzzles with the 7-8 PDB) on a uniform random sample of the timed stream,
+    one query per host thread. The sample is a prefix of a seeded random permutation of
+    the stream, long enough that its predicted single-thread work fills about budget_s
+    seconds on `threads` threads. A puzzle whose A* needs more than puzzle_cap expansions
+    (its count is in the c3 fixture) runs to puzzle_cap, and its time is scaled up by its
+    expansion count; such times are marked extrapolated. `value` is the stream's estimated
+    CPU throughput: threads / (mean single-thread time per query of the sample)."""
+    from oracle import oracle as O
+    import hashlib
+    threads = threads or min(16, os.cpu_count() or 1)
+    # the 15-puzzle PDB: the device-built table, byte-equal to the oracle's BFS (checked
+    # here against the golden digest; the CPU BFS takes minutes)
+    pdb = ctx.puzzle_download_pdb(4)
+    with open(os.path.join(ROOT, "tests", "golden", "pdb_digests.json")) as f:
+        digests = json.load(f)["4"]
+    off = 0
+    for d in digests:
+        assert hashlib.sha256(pdb[off: off + d["size"]].tobytes()).hexdigest() == d["sha256"], "PDB digest"
+        off += d["size"]
+    rng = np.random.default_rng(seed)
+    order = [int(i) for i in rng.permutation(len(specs))]
+
+    def predicted_s(s):  # rough single-thread seconds, only to size the sample
+        if s["kind"] == "grid":
+            return 1.2e-7 * s["H"] * s["W"] + 0.05
+        return 3.5e-6 * min(s["expanded"], puzzle_cap)
+
+    sample, work = [], 0.0
+    for i in order:
+        if len(sample) >= threads and work >= budget_s * threads:
+            break
+        sample.append(i)
+        work += predicted_s(specs[i])
+
+    def solve(i):
+        s = specs[i]
         if s["kind"] == "grid":
             g = O.grid_bernoulli(s["H"], s["W"], s["seed"], s["p"], *s["s"], *s["g"])
             t = time.perf_counter()
             O.grid_astar(g, s["H"], s["W"], *s["s"], *s["g"], "f32", want_path=False)
-        else:
-            t = time.perf_counter()
-            O.puzzle_solve(4, np.array(s["board"], np.uint8), pdb, "astar")
-        return time.perf_counter() - t
+            return time.perf_counter() - t, False
+        t = time.perf_counter()
+        capped = s["expanded"] > puzzle_cap
+        O.puzzle_solve(4, np.array(s["board"], np.uint8), pdb, "astar", node_limit=puzzle_cap if capped else 0)
+        dt = time.perf_counter() - t
+        return (dt * s["expanded"] / puzzle_cap, True) if capped else (dt, False)
 
-    if any(s["kind"] == "puzzle" for s in sample):
-        pdb = O.pdb_full(4, 1, cache_dir=os.path.join(ROOT, "_cache"))
     t = time.perf_counter()
     with cf.ThreadPoolExecutor(threads) as ex:
-        times = list(ex.map(solve, sample))
+        out = list(ex.map(solve, sample))
     wall = time.perf_counter() - t
-    return {"value": len(sample) / wall, "unit": "queries/s", "cores": threads, "kind": "port",
-            "sample": f"{len(sample)} queries of the stream (grids with side <= 2048, puzzles with <= 1.5M "
-                      f"A* expansions) on {threads} threads in {wall:.1f} s; mean single-thread solve "
-                      f"{np.mean(times) * 1e3:.0f} ms (PDB build and input generation excluded)"}
+    times = [x[0] for x in out]
+    n_ext = sum(x[1] for x in out)
+    mean = float(np.mean(times))
+    model, nproc = host_cpu()
+    n_grid = sum(specs[i]["kind"] == "grid" for i in sample)
+    return {"value": threads / mean, "unit": "queries/s", "cores": threads, "kind": "port",
+            "nproc": nproc, "cpu_model": model,
+            "sample": f"uniform random sample of {len(sample)} of the {len(specs)} timed stream queries "
+                      f"({n_grid} grids, {len(sample) - n_grid} puzzles), solved by the oracle (FP32 grid costs, "
+                      f"reference containers) one query per thread on {threads} threads in {wall:.1f} s; value = "
+                      f"{threads} threads / mean single-thread time per query ({mean * 1e3:.0f} ms), i.e. the "
+                      f"stream's CPU throughput with perfect packing; {n_ext} puzzle time(s) extrapolated from "
+                      f"{puzzle_cap} expansions by their A* expansion counts; PDB build and input generation "
+                      f"excluded",
+            "extrapolated_stream_s": mean * len(specs) / threads,
+            "measured_sample_qps": len(sample) / wall}
 
 
 def run_serve_mixed(args):
     import uastar_amd as U
     ctx = U.Context(0, graphs=args.graph)
-    specs = mixed_stream(args.steps + args.warmup)
+    n_warm, n_timed = args.warmup * QUERIES_PER_STEP, args.steps * QUERIES_PER_STEP
+    specs = mixed_stream(n_warm + n_timed)
     materialize(ctx, specs)
-    warm, specs = specs[: args.warmup], specs[args.warmup:]
+    warm, specs = specs[:n_warm], specs[n_warm:]
     ctx.puzzle_build_pdb(4)
     tuning = None
     if not args.no_tune:
@@ -324,22 +391,28 @@ def run_serve_mixed(args):
     launches = st1["launches"] - st0["launches"]
     alg = sum(alg_bytes(r, s) for r, s in zip(sres, specs))
     achieved = alg / (kms * 1e-3) / 1e9 if kms > 0 else 0.0  # = per-launch bytes / average launch time
+    atomics = sum(r["atomics"] for r in sres)
+    bench_args = f"--steps {args.steps} --warmup {args.warmup} --load {args.load:g}"
+    traffic, traffic_src = measured_traffic(bench_args)
     out = {
         "metric": "queries/s + p50/p99 latency, batched serving vs plain GA*, 1 MI355X, exact",
         "value": serve_qps,
         "unit": "queries/s",
         "n_gpus": 1,
-        "steps": len(specs),
-        "warmup": len(warm),
-        "ms_per_step": span / len(specs),
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": span / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "int64",
         "data": "synthetic (seeded generators)",
-        "config": {"workload": "serving stream: 70% grids (side log-uniform in [512, 8192], Bernoulli(p) "
-                               "obstacles with p ~ U[0.2,0.4], random open start/goal), 30% random solvable "
-                               "15-puzzles with the 7-8 PDB; Poisson arrivals at load x plain capacity",
+        "config": {"workload": f"serving stream of {len(specs)} queries ({args.steps} steps of "
+                               f"{QUERIES_PER_STEP} arrivals, after {len(warm)} warm-up queries): 70% grids (side "
+                               "log-uniform in [512, 8192], Bernoulli(p) obstacles with p ~ U[0.2,0.4], random open "
+                               "start/goal), 30% random solvable 15-puzzles with the 7-8 PDB; Poisson arrivals at "
+                               "load x plain capacity",
+                   "queries": len(specs), "queries_per_step": QUERIES_PER_STEP,
                    "mode": "serve (continuous batching; team width per class from the autotuner)",
                    "load": args.load,
                    "arrival_rate_qps": rate},
@@ -365,10 +438,14 @@ def run_serve_mixed(args):
                    "graph_error": ls1["graph_error"]},
         "tuning": tuning,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
                      "kernel": "k_teams<2, 0> (serving step, grids and 15-puzzles)", "launches": launches,
                      "kernel_ms_avg": kms / max(1, launches), "alg_bytes_per_launch": alg / max(1, launches),
                      "kernel_ms_total": kms, "alg_bytes_total": alg},
+        # device-memory atomics on the search state (relaxing atomicMin on best words and
+        # hash values, hash-claiming CAS), counted by the kernel, over the serving launches
+        "atomics": {"total": atomics, "per_launch": atomics / max(1, launches),
+                    "per_s": atomics / (kms * 1e-3) if kms > 0 else 0.0},
     }
     def diag(res, lat=None):
         rows = []
@@ -395,8 +472,8 @@ def run_serve_mixed(args):
     for x in sorted(srows, key=lambda x: -(x["lat_ms"] or 0))[:8]:
         say(f"slowest served: {x}")
     if not args.no_cpu:
-        say("CPU baseline (oracle, bounded sample)")
-        out["cpu_baseline"] = cpu_baseline_mixed(specs)
+        say("CPU baseline (oracle, uniform sample of the stream)")
+        out["cpu_baseline"] = cpu_baseline_mixed(ctx, specs)
     for s in specs + warm:
         if s["kind"] == "grid":
             s["grid"].free()
@@ -462,8 +539,9 @@ def run_deep_grid(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--steps", type=int, default=20,
+                    help="serving stream: batches of %d arrivals; plain workloads: queries" % QUERIES_PER_STEP)
+    ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="serve-mixed", choices=["serve-mixed", "plain-grid1000", "plain-grid16384", "deep-grid"])
     ap.add_argument("--load", type=float, default=20.0, help="arrival rate as a multiple of plain capacity")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")

@@ -55,6 +55,9 @@ typedef struct {
 /* ua_config.flags: replay each search launch (copy in, cooperative kernel, copy out)
  * from an instantiated hipGraph instead of issuing it call by call */
 #define UA_FLAG_GRAPH 1
+/* ua_config.flags: a served query that finds its goal keeps its path (grid cell ids) or
+ * moves (puzzle) on the host until ua_serve_path fetches them */
+#define UA_FLAG_SERVE_PATHS 2
 
 typedef struct {
     int32_t status;
@@ -69,6 +72,8 @@ typedef struct {
     double latency_ms;   /* serve: completion - submission; batch: completion - launch */
     int32_t optimal;     /* 1: the cost is proven optimal (every exact search that finds a path) */
     int32_t weight;      /* weight on h of the search that found the answer (1 = plain A*) */
+    int64_t atomics;     /* device-memory atomics the search issued on its table or hash (relaxing
+                            atomicMin, slot-claiming CAS) */
 } ua_result;
 
 /* Device context: owns every device allocation. Replaces the device set-up of
@@ -80,12 +85,17 @@ const char *ua_last_error(ua_ctx *ctx);
 /* 1 if a GPU is visible, without creating a context */
 int ua_device_count(void);
 
-/* ---- grids. Replace the H2D copy of Pathway::graph() (pathway/GPU-solver.cu:140). */
+/* ---- grids. Replace the H2D copy of Pathway::graph() (pathway/GPU-solver.cu:103). */
 int ua_grid_create(ua_ctx *ctx, int H, int W, const uint64_t *host_bitmap, ua_grid **out);
-/* from the byte-per-cell graph of Pathway::graph() (0 = blocked, nonzero = open) */
+/* From the byte-per-cell graph of Pathway::graph() (pathway.hpp:116-119): 0 = blocked,
+ * 0xFF = open. A byte is a cell's out-move mask (bit d gates move d,
+ * pathway/CPU-solver.cpp:64-66); every input module writes all-or-nothing masks
+ * (input/custom.cpp:23, input/zigzag.cpp:26-39), and any other byte is rejected with
+ * UA_ERR_INVALID (the grid is not created). */
 int ua_grid_from_bytes(ua_ctx *ctx, int H, int W, const uint8_t *bytes, ua_grid **out);
 /* i.i.d. Bernoulli(block_rate) obstacles from seed, generated on the device;
- * start and goal forced open (the advertised `random` input module, main.cpp:279-281) */
+ * start and goal forced open (the advertised `random` input module, main.cpp:28-29 and
+ * :129-130, which the reference never built) */
 int ua_grid_generate(ua_ctx *ctx, int H, int W, uint64_t seed, double block_rate, int sx, int sy,
                      int ex, int ey, ua_grid **out);
 int ua_grid_download(ua_ctx *ctx, const ua_grid *g, uint64_t *host_bitmap);
@@ -144,6 +154,12 @@ int ua_serve_submit(ua_ctx *ctx, const ua_query *q, uint64_t *ticket);
 int ua_serve_step(ua_ctx *ctx, int64_t *n_active, int64_t *n_pending);
 /* up to cap finished queries: their tickets and results */
 int ua_serve_poll(ua_ctx *ctx, uint64_t *tickets, ua_result *results, int64_t cap, int64_t *n_out);
+/* Path of a served query (start..goal cell ids, or blank moves 0..3), kept when the
+ * context has UA_FLAG_SERVE_PATHS: *len gets its length; with out == NULL only the length
+ * is returned, otherwise it is copied out and released. The served counterpart of
+ * GPUPathwaySolver::getSolution / GPUPuzzleSolver<N>::getSolution (pathway/GPU-solver.cu:
+ * 292-316, puzzle/GPU-solver.cuh:276-287). */
+int ua_serve_path(ua_ctx *ctx, uint64_t ticket, int32_t *out, int64_t cap, int64_t *len);
 /* steps until nothing is pending or active */
 int ua_serve_drain(ua_ctx *ctx);
 /* launches so far, their summed duration (HIP events), the device-time budget of one

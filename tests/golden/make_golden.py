@@ -6,6 +6,7 @@
     python tests/golden/make_golden.py grids    # seeded grid configs (1000^2 seed 1; 16384^2 seed 2)
     python tests/golden/make_golden.py pdb5     # the four 6-tile 5x5 PDB digests (added to pdb_digests.json)
     python tests/golden/make_golden.py c5       # 5x5 boards from seeded random walks off the goal
+    python tests/golden/make_golden.py c5stream # queries of the serving stream (seed 5) with dense-oracle answers
 
 The oracle itself is pinned by tests/test_oracle.py (closed forms, exhaustive BFS,
 textbook Dijkstra / IDA*); these files record its answers at sizes the GPU tests
@@ -138,6 +139,45 @@ def cmd_grids():
         json.dump(rows, f, indent=1)
 
 
+def _solve_stream_grid(s):
+    g = O.grid_bernoulli(s["H"], s["W"], s["seed"], s["p"], *s["s"], *s["g"])
+    t = time.time()
+    r = O.grid_astar(g, s["H"], s["W"], *s["s"], *s["g"], "dense", want_path=False)
+    return {"found": bool(r["found"]), "orth": int(r["orth"]), "diag": int(r["diag"]),
+            "expanded": int(r["expanded"]), "cpu_s": round(time.time() - t, 2)}
+
+
+def cmd_c5stream(workers=6, first=40, deepest=3):
+    """Queries of the serving stream of the bench (bench.mixed_stream, seed 5, 204 queries
+    in stream order: 70 % grids with side log-uniform in [512, 8192], 30 % 15-puzzles
+    from the c3 boards): the first `first` of them plus the `deepest` grids of the
+    whole stream (largest start-goal Chebyshev distance). Grid answers come from the
+    oracle's dense exact A*; puzzle answers are the c3 fixture's."""
+    import bench
+    specs = bench.mixed_stream(204)
+    grids = [i for i, s in enumerate(specs) if s["kind"] == "grid"]
+    cheb = lambda s: max(abs(s["s"][0] - s["g"][0]), abs(s["s"][1] - s["g"][1]))
+    deep = sorted(grids, key=lambda i: -cheb(specs[i]))[:deepest]
+    pick = sorted(set(range(first)) | set(deep))
+    todo = [specs[i] for i in pick if specs[i]["kind"] == "grid"]
+    with mp.Pool(workers) as pool:
+        answers = pool.map(_solve_stream_grid, todo, chunksize=1)
+    it = iter(answers)
+    rows = []
+    for i in pick:
+        s = specs[i]
+        if s["kind"] == "grid":
+            a = next(it)
+            rows.append({"index": i, "kind": "grid", "H": s["H"], "W": s["W"], "seed": s["seed"], "p": s["p"],
+                         "start": list(s["s"]), "goal": list(s["g"]), "deep": i in deep, **a})
+        else:
+            rows.append({"index": i, "kind": "puzzle", "board": s["board"], "steps": s["steps"],
+                         "expanded": s["expanded"]})
+        print(rows[-1] if s["kind"] == "puzzle" else {k: v for k, v in rows[-1].items() if k != "seed"}, flush=True)
+    with open(os.path.join(HERE, "c5_stream.json"), "w") as f:
+        json.dump(rows, f, indent=0)
+
+
 if __name__ == "__main__":
     {"pdb": cmd_pdb, "pdb5": lambda: cmd_pdb((5,)), "c3": cmd_c3, "c4": cmd_c4, "grids": cmd_grids,
-     "c5": cmd_c5}[sys.argv[1]]()
+     "c5": cmd_c5, "c5stream": cmd_c5stream}[sys.argv[1]]()

@@ -25,8 +25,18 @@ def run(args, stdin="", cwd=None, timeout=300):
 
 
 def costs(out, who):
-    m = re.search(rf"Optimal distance from {who}: [0-9.]+ \((\d+) \+ (\d+) \* sqrt\(2\)\)", out)
-    return (int(m.group(1)), int(m.group(2))) if m else None
+    """The exact (orth, diag) pair of the run, checked against the reference's own line
+    ' > Optimal distance from WHO: %.3f' (pathway/pathway.cpp:115-119), which must be
+    present, exactly so formatted, and agree with the pair."""
+    m = re.search(rf"^ > Exact distance from {who}: (\d+) \+ (\d+) \* sqrt\(2\)$", out, re.M)
+    if not m:
+        assert not re.search(rf"Optimal distance from {who}", out), out
+        return None
+    pair = (int(m.group(1)), int(m.group(2)))
+    line = re.search(rf"^ > Optimal distance from {who}: (\d+\.\d{{3}})$", out, re.M)
+    assert line, out
+    assert line.group(1) == f"{pair[0] + pair[1] * 2 ** 0.5:.3f}"
+    return pair
 
 
 def test_readme_example(tmp_path):
@@ -34,7 +44,10 @@ def test_readme_example(tmp_path):
     rc, out = run(["--pathway", "-H", "5", "-W", "5", "--input-module", "custom", "--no-gpu"], grid, cwd=tmp_path)
     assert rc == 0, out
     assert costs(out, "CPU") == (4, 3)
-    assert "8.243" in out
+    assert " > Optimal distance from CPU: 8.243\n" in out
+    # the lines of Pathway::printSolution and CPUPathwaySolver::getSolution
+    assert " > Number of -: 4, +: 3\n" in out
+    assert re.search(r"^\t\t\tNumber of nodes expanded: \d+$", out, re.M)
     cells = re.findall(r"\((\d+) (\d+)\)", (tmp_path / "solutionCPU.txt").read_text())
     assert cells[0] == ("0", "0") and cells[-1] == ("4", "4") and len(cells) == 8
 
@@ -68,7 +81,8 @@ def test_puzzle3_cache_file_and_solution(oracle, tmp_path):
     steps, _, _ = oracle.puzzle_solve(3, np.array(board, np.uint8), oracle.pdb_full(3), "astar")
     rc, out = run(["--puzzle", "-W", "3", "-C", "--pdb-dir", str(tmp_path)], " ".join(map(str, board)), cwd=tmp_path)
     assert rc == 0, out
-    assert f"Optimal steps from CPU: {steps}" in out
+    assert f"\n > Optimal steps from CPU: {steps}\n" in out
+    assert re.search(r"^\t\tNumber of nodes deduplicated: \d+\n\t\tNumber of nodes expanded: \d+$", out, re.M)
     digest = json.load(open(os.path.join(GOLDEN, "pdb_digests.json")))["3"][0]
     name = "database_3_" + "_".join(map(str, digest["tiles"])) + ".bin"
     assert hashlib.sha256((tmp_path / name).read_bytes()).hexdigest() == digest["sha256"]
@@ -110,6 +124,7 @@ def test_cli_cpu_and_gpu_agree(tmp_path):
                   cwd=tmp_path)
     assert rc == 0, out
     assert costs(out, "CPU") == costs(out, "GPU") and costs(out, "GPU") is not None
+    assert re.search(r"^\t\t\t Number of nodes expanded: \d+$", out, re.M), out
     gpu_cells = re.findall(r"\((\d+) (\d+)\)", (tmp_path / "solutionGPU.txt").read_text())
     assert gpu_cells[0] == ("0", "0") and gpu_cells[-1] == ("399", "299")
     print("cli: 8-puzzle", flush=True)

@@ -27,13 +27,13 @@ class QueryDesc(C.Structure):
         ("start_state", C.c_uint64), ("start_state2", C.c_uint64), ("pdb", C.c_void_p), ("resume", C.c_void_p),
         ("round_budget", C.c_int32), ("cls", C.c_int32), ("wshift", C.c_int32), ("bounded", C.c_int32)]
 
-RESUME_WORDS_4 = 8 + 4 * 130
+RESUME_WORDS_4 = 9 + 4 * 130
 
 
 class QueryResult(C.Structure):
     _fields_ = [("status", C.c_int32), ("found", C.c_int32), ("goal", C.c_uint64), ("rounds", C.c_int64),
                 ("pops", C.c_int64), ("expanded", C.c_int64), ("generated", C.c_int64), ("inserted", C.c_int64),
-                ("flags", C.c_uint32), ("pad", C.c_int32), ("t_start", C.c_int64), ("t_end", C.c_int64)]
+                ("atomics", C.c_int64), ("flags", C.c_uint32), ("pad", C.c_int32), ("t_start", C.c_int64), ("t_end", C.c_int64)]
 
 
 @pytest.fixture(scope="module")

@@ -97,6 +97,27 @@ def test_edge_cases(ctx, oracle):
         _assert_parity(g, H, W, sx, sy, ex, ey, ref, res, path)
 
 
+def test_partial_out_move_masks_are_rejected(ctx):
+    """A cell byte other than 0 and 0xFF gates single moves (pathway/CPU-solver.cpp:64-66),
+    which the 1-bit grid cannot hold: ua_grid_from_bytes refuses the grid (UA_ERR_INVALID)
+    instead of treating the cell as fully open; 0 / 0xFF bytes still load."""
+    H, W = 9, 70
+    g = np.full(H * W, 0xFF, np.uint8)
+    g[3 * W + 66] = 0x0F
+    _say("partial mask 0x0F at (3, 66)")
+    with pytest.raises(U.UAError) as e:
+        ctx.grid_from_bytes(g, H, W)
+    assert e.value.status == -1 and "partial out-move mask" in str(e.value)
+    g[3 * W + 66] = 0x80
+    with pytest.raises(U.UAError):
+        ctx.grid_from_bytes(g, H, W)
+    g[3 * W + 66] = 0
+    grid = ctx.grid_from_bytes(g, H, W)
+    r = ctx.grid_solve(grid, 0, 0, H - 1, W - 1)
+    grid.free()
+    assert r["found"] and (r["orth"], r["diag"]) == (61, 8)
+
+
 @pytest.mark.parametrize("seed", range(12))
 def test_random_small(ctx, oracle, seed):
     rng = np.random.default_rng(100 + seed)

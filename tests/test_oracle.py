@@ -196,6 +196,18 @@ def test_pdb3_equals_exhaustive_bfs(oracle, bfs8):
             assert p[oracle.pdb_encode(3, tracked, np.array(s, np.uint8))] == d
 
 
+def test_solvable_rule_is_reachability_over_all_eight_puzzle_boards(oracle, bfs8):
+    """orc_puzzle_solvable (the parity rule of the generators) against the exhaustive BFS:
+    over all 9! boards it holds exactly for the 181,440 reachable from the goal."""
+    import itertools
+    n_true = 0
+    for perm in itertools.permutations(range(9)):
+        ok = oracle.puzzle_solvable(3, np.array(perm, np.uint8))
+        assert ok == (perm in bfs8), perm
+        n_true += ok
+    assert n_true == 181440
+
+
 def test_eight_puzzle_astar_vs_bfs(oracle, bfs8):
     pdb = oracle.pdb_full(3)
     rng = np.random.default_rng(3)

@@ -42,6 +42,7 @@ class Config(C.Structure):
 
 
 FLAG_GRAPH = 1
+FLAG_SERVE_PATHS = 2
 
 
 class Result(C.Structure):
@@ -49,7 +50,7 @@ class Result(C.Structure):
                 ("steps", C.c_int64), ("cost", C.c_double), ("rounds", C.c_int64), ("pops", C.c_int64),
                 ("expanded", C.c_int64), ("generated", C.c_int64), ("inserted", C.c_int64),
                 ("device_ms", C.c_double), ("kernel_ms", C.c_double), ("latency_ms", C.c_double),
-                ("optimal", C.c_int32), ("weight", C.c_int32)]
+                ("optimal", C.c_int32), ("weight", C.c_int32), ("atomics", C.c_int64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -108,6 +109,7 @@ def lib():
             "ua_serve_step": (C.c_int, [vp, C.POINTER(i64), C.POINTER(i64)]),
             "ua_serve_poll": (C.c_int, [vp, C.POINTER(u64), C.POINTER(Result), i64, C.POINTER(i64)]),
             "ua_serve_drain": (C.c_int, [vp]),
+            "ua_serve_path": (C.c_int, [vp, u64, vp, i64, C.POINTER(i64)]),
             "ua_serve_stats": (C.c_int, [vp, C.POINTER(i64), C.POINTER(C.c_double), C.POINTER(i32),
                                          C.POINTER(i64)]),
             "ua_launch_stats": (C.c_int, [vp, C.POINTER(i64), C.POINTER(i64), C.POINTER(C.c_double),
@@ -199,8 +201,8 @@ class Context:
     """One device context (ua_ctx)."""
 
     def __init__(self, ordinal=0, team_waves=0, plain_heaps=0, max_rounds=0, hash_log2=0, mem_budget=0,
-                 puzzle_team_waves=0, graphs=False):
-        flags = FLAG_GRAPH if graphs else 0
+                 puzzle_team_waves=0, graphs=False, serve_paths=False):
+        flags = (FLAG_GRAPH if graphs else 0) | (FLAG_SERVE_PATHS if serve_paths else 0)
         self.cfg = Config(team_waves, plain_heaps, max_rounds, hash_log2, mem_budget, puzzle_team_waves, flags)
         h = C.c_void_p()
         rc = lib().ua_create(ordinal, C.byref(self.cfg), C.byref(h))
@@ -340,6 +342,15 @@ class Context:
 
     def serve_drain(self):
         self._check(lib().ua_serve_drain(self.h))
+
+    def serve_path(self, ticket: int) -> np.ndarray:
+        """Cell ids (grid) or blank moves (puzzle) of a served query that found its goal
+        (contexts made with serve_paths=True); released once fetched."""
+        n = C.c_int64()
+        self._check(lib().ua_serve_path(self.h, C.c_uint64(ticket), None, 0, C.byref(n)))
+        out = np.empty(max(1, n.value), np.int32)
+        self._check(lib().ua_serve_path(self.h, C.c_uint64(ticket), out.ctypes.data, n.value, C.byref(n)))
+        return out[: n.value]
 
     def serve_stats(self):
         n, ms, rb, wd = C.c_int64(), C.c_double(), C.c_int32(), C.c_int64()

@@ -33,8 +33,9 @@ uint64_t octile(int x, int y, int ex, int ey) {
 double GridCost::value() const { return (double)orth + (double)diag * std::sqrt(2.0); }
 
 bool cpu_grid_astar(const std::vector<uint8_t> &graph, int H, int W, int sx, int sy, int ex, int ey,
-                    GridCost *cost, std::vector<int> *path) {
+                    GridCost *cost, std::vector<int> *path, int64_t *expanded) {
     const size_t N = (size_t)H * W;
+    int64_t n_exp = 0;
     std::vector<uint64_t> g(N, ~0ull);
     std::vector<uint8_t> via(N, 0xFF);
     using Item = std::pair<uint64_t, uint32_t>;  // (f key, cell)
@@ -52,6 +53,7 @@ bool cpu_grid_astar(const std::vector<uint8_t> &graph, int H, int W, int sx, int
             found = true;
             break;
         }
+        ++n_exp;
         for (int d = 0; d < 8; ++d) {
             if (!(graph[u] & (1u << d))) continue;
             int nx = x + DX[d], ny = y + DY[d];
@@ -64,6 +66,7 @@ bool cpu_grid_astar(const std::vector<uint8_t> &graph, int H, int W, int sx, int
             open.push({ng + octile(nx, ny, ex, ey), v});
         }
     }
+    if (expanded) *expanded = n_exp;
     if (!found) return false;
     std::vector<int> rev;
     int64_t orth = 0, diag = 0;
@@ -273,8 +276,16 @@ bool puzzle_solvable(int n, const std::vector<uint8_t> &board) {
 }
 
 bool cpu_puzzle_astar(int n, const std::vector<uint8_t> &board, const std::vector<uint8_t> &pdb, int *steps,
-                      std::vector<int> *moves) {
-    if (!puzzle_solvable(n, board)) return false;
+                      std::vector<int> *moves, int64_t *expanded, int64_t *dedup) {
+    int64_t n_exp = 0, n_dup = 0;
+    auto counts = [&] {
+        if (expanded) *expanded = n_exp;
+        if (dedup) *dedup = n_dup;
+    };
+    if (!puzzle_solvable(n, board)) {
+        counts();
+        return false;
+    }
     PuzzleCtx ctx{n, &pdb, pdb_partitions(n), {}};
     size_t off = 0;
     for (const auto &t : ctx.parts) {
@@ -319,8 +330,10 @@ bool cpu_puzzle_astar(int n, const std::vector<uint8_t> &board, const std::vecto
             }
             *steps = g;
             if (moves) moves->assign(rev.rbegin(), rev.rend());
+            counts();
             return true;
         }
+        ++n_exp;
         cells_of(s, ct);
         int bl = ct[0], bx = bl / n, by = bl % n;
         for (int d = 0; d < 4; ++d) {
@@ -331,7 +344,10 @@ bool cpu_puzzle_astar(int n, const std::vector<uint8_t> &board, const std::vecto
             unsigned t = tile_at(s, nb);
             Board ns = s + ((Board)t << (5 * bl)) - ((Board)t << (5 * nb));
             auto it = seen.find(ns);
-            if (it != seen.end() && it->second.g <= g + 1) continue;
+            if (it != seen.end() && it->second.g <= g + 1) {
+                ++n_dup;
+                continue;
+            }
             seen[ns] = {(uint8_t)(g + 1), (uint8_t)d};
             uint8_t nct[25];
             std::copy(ct, ct + 25, nct);
@@ -340,6 +356,7 @@ bool cpu_puzzle_astar(int n, const std::vector<uint8_t> &board, const std::vecto
             open.push({g + 1 + ctx.h(nct), -(g + 1), ns});
         }
     }
+    counts();
     return false;
 }
 

@@ -16,8 +16,9 @@ struct GridCost {
 
 // 8-direction grid, one byte per cell (nonzero = open; bit d = move d allowed out of
 // the cell). A move u -> v needs bit d of u and v in range; returns false without a path.
+// *expanded (if given) receives the number of nodes expanded.
 bool cpu_grid_astar(const std::vector<uint8_t> &graph, int H, int W, int sx, int sy, int ex, int ey,
-                    GridCost *cost, std::vector<int> *path);
+                    GridCost *cost, std::vector<int> *path, int64_t *expanded = nullptr);
 
 // The additive disjoint partitions for an n x n board (n = 3, 4 or 5).
 std::vector<std::vector<int>> pdb_partitions(int n);
@@ -32,9 +33,10 @@ std::vector<uint8_t> pdb_load_or_build(int n, const std::string &dir, bool write
 bool pdb_write_cache(int n, const std::string &dir, const std::vector<uint8_t> &pdb);
 
 // Optimal blank moves (0..3 = (+1,0), (-1,0), (0,+1), (0,-1)) of a board (row-major,
-// 0 = blank); false when the board is not solvable.
+// 0 = blank); false when the board is not solvable. *expanded and *dedup (if given)
+// receive the nodes expanded and the successors dropped as already seen at no larger g.
 bool cpu_puzzle_astar(int n, const std::vector<uint8_t> &board, const std::vector<uint8_t> &pdb, int *steps,
-                      std::vector<int> *moves);
+                      std::vector<int> *moves, int64_t *expanded = nullptr, int64_t *dedup = nullptr);
 bool puzzle_solvable(int n, const std::vector<uint8_t> &board);
 
 }  // namespace cli

@@ -81,7 +81,7 @@ public:
     }
 
     void cpuSolve() override {
-        cpu_ok_ = cpu_grid_astar(graph_, H_, W_, sx_, sy_, ex_, ey_, &cpu_cost_, &cpu_path_);
+        cpu_ok_ = cpu_grid_astar(graph_, H_, W_, sx_, sy_, ex_, ey_, &cpu_cost_, &cpu_path_, &cpu_expanded_);
         cpu_done_ = true;
     }
 
@@ -94,6 +94,8 @@ public:
         gpu_ok_ = rc == UA_OK;
         gpu_done_ = true;
         if (gpu_ok_) {
+            // the line GPUPathwaySolver::solve prints at termination (pathway/GPU-solver.cu:200)
+            std::printf("\t\t\t Number of nodes expanded: %d\n", (int)res_.expanded);
             int64_t len = 0;
             if (ua_grid_path(ctx_, nullptr, 0, &len) || len < 1) return false;
             std::vector<int32_t> ids(len);
@@ -105,14 +107,27 @@ public:
         return true;
     }
 
+    // The order and text of Pathway::output (pathway/pathway.cpp:89-130): the solution files
+    // with their "-" / "+" counts (the CPU solver's getSolution first prints its node
+    // count, pathway/CPU-solver.cpp:94), then the optimal distances. The exact step counts
+    // follow on lines of their own.
     bool output() override {
         if (cpu_done_ && gpu_done_ && cpu_ok_ != gpu_ok_) return false;
-        if (cpu_done_) report(cpu_ok_, cpu_path_, cpu_cost_, "CPU", "solutionCPU.txt");
-        if (gpu_done_) {
-            report(gpu_ok_, gpu_path_, gpu_cost_, "GPU", "solutionGPU.txt");
-            if (gpu_ok_)
-                std::printf(" > GPU search: %lld rounds, %lld nodes expanded, %.2f ms on the device\n",
-                            (long long)res_.rounds, (long long)res_.expanded, res_.device_ms);
+        if (cpu_done_) {
+            if (cpu_ok_) std::printf("\t\t\tNumber of nodes expanded: %d\n", (int)cpu_expanded_);
+            report(cpu_ok_, cpu_path_, cpu_cost_, "CPU", "solutionCPU.txt");
+        }
+        if (gpu_done_) report(gpu_ok_, gpu_path_, gpu_cost_, "GPU", "solutionGPU.txt");
+        if (cpu_done_ && cpu_ok_) std::printf(" > Optimal distance from CPU: %.3f\n", cpu_cost_.value());
+        if (gpu_done_ && gpu_ok_) std::printf(" > Optimal distance from GPU: %.3f\n", gpu_cost_.value());
+        if (cpu_done_ && cpu_ok_)
+            std::printf(" > Exact distance from CPU: %lld + %lld * sqrt(2)\n", (long long)cpu_cost_.orth,
+                        (long long)cpu_cost_.diag);
+        if (gpu_done_ && gpu_ok_) {
+            std::printf(" > Exact distance from GPU: %lld + %lld * sqrt(2)\n", (long long)gpu_cost_.orth,
+                        (long long)gpu_cost_.diag);
+            std::printf(" > GPU search: %lld rounds, %lld nodes expanded, %.2f ms on the device\n",
+                        (long long)res_.rounds, (long long)res_.expanded, res_.device_ms);
         }
         if (!o_.plot.empty()) plot(gpu_done_ && gpu_ok_ ? gpu_path_ : cpu_path_);
         // exact comparison: the (orth, diag) step counts are the cost
@@ -145,8 +160,6 @@ private:
             std::printf("ERROR: %s cannot be open for writing.\n", file);
         }
         std::printf(" > Number of -: %lld, +: %lld\n", (long long)c.orth, (long long)c.diag);
-        std::printf(" > Optimal distance from %s: %.3f (%lld + %lld * sqrt(2))\n", who, c.value(),
-                    (long long)c.orth, (long long)c.diag);
     }
 
     // A 24-bit BMP with one px-by-px square per cell: open white, blocked grey, path green.
@@ -196,6 +209,7 @@ private:
     ua_result res_{};
     bool cpu_done_ = false, gpu_done_ = false, cpu_ok_ = false, gpu_ok_ = false;
     GridCost cpu_cost_, gpu_cost_;
+    int64_t cpu_expanded_ = 0;
     std::vector<int> cpu_path_, gpu_path_;
 };
 

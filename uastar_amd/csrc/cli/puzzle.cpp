@@ -68,8 +68,13 @@ public:
     }
 
     void cpuSolve() override {
-        cpu_ok_ = cpu_puzzle_astar(n_, board_, pdb_, &cpu_steps_, &cpu_moves_);
+        int64_t expanded = 0, dedup = 0;
+        cpu_ok_ = cpu_puzzle_astar(n_, board_, pdb_, &cpu_steps_, &cpu_moves_, &expanded, &dedup);
         cpu_done_ = true;
+        if (cpu_ok_) {  // the lines CPUPuzzleSolver::solve prints at the goal (puzzle/CPU-solver.hpp:169-170)
+            std::printf("\t\tNumber of nodes deduplicated: %d\n", (int)dedup);
+            std::printf("\t\tNumber of nodes expanded: %d\n", (int)expanded);
+        }
     }
 
     bool gpuSolve() override {
@@ -81,6 +86,8 @@ public:
         gpu_ok_ = rc == UA_OK;
         gpu_done_ = true;
         if (gpu_ok_) {
+            // the line GPUPuzzleSolver<N>::solve prints at termination (puzzle/GPU-solver.cuh:246)
+            std::printf("\t\t\t Number of nodes expanded: %d\n", (int)res_.expanded);
             int64_t len = 0;
             if (ua_puzzle_moves(ctx_, nullptr, 0, &len)) return false;
             std::vector<int32_t> m(len > 0 ? len : 1);

@@ -171,7 +171,7 @@ struct StepHeader {
 
 // Resume record of a suspended query: header, then per heap its root batch,
 // partial batch and (n, pc). Written when a serving launch's round budget runs out.
-constexpr int RESUME_HEADER = 8;   // valid, round, best_goal, pops, expanded, generated, inserted, flags
+constexpr int RESUME_HEADER = 9;   // valid, round, best_goal, pops, expanded, generated, inserted, flags, atomics
 constexpr int RESUME_PER_HEAP = 130;
 UA_HD inline int64_t resume_words(int heaps) { return RESUME_HEADER + (int64_t)heaps * RESUME_PER_HEAP; }
 
@@ -180,7 +180,7 @@ struct QueryResult {
     int32_t found;
     uint64_t goal;       // grid: goal g key; puzzle: (g << 32) | goal slot
     int64_t rounds;
-    int64_t pops, expanded, generated, inserted;
+    int64_t pops, expanded, generated, inserted, atomics;
     uint32_t flags;
     int32_t pad;
     int64_t t_start, t_end;   // device clock (s_memrealtime, 100 MHz) for latency
@@ -202,7 +202,7 @@ struct TeamVars {
     uint32_t bar_gen;
     uint32_t abort;
     uint32_t next_q;     // query index handed from the group leader to the group
-    uint64_t n_pop, n_exp, n_gen, n_ins;
+    uint64_t n_pop, n_exp, n_gen, n_ins, n_atom;
     uint64_t deadline;   // serving: device time at which the group's budget ends (0 = unset)
 };
 

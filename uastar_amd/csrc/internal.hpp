@@ -154,6 +154,9 @@ void pack_board5(const uint8_t *t, uint64_t *w0, uint64_t *w1);
 int puzzle_fill_desc(ua_ctx *c, int n, const uint8_t *tiles, int heaps, int hash_log2, QueryDesc &q, Plan &p);
 int ensure_pdb(ua_ctx *c, int n);
 void fill_result(const QueryDesc &q, const QueryResult &r, ua_result *o);
+// start..goal cell ids (grid) or blank moves (puzzle) of a finished query whose state is
+// still on the device
+int fetch_path(ua_ctx *c, const QueryDesc &q, const QueryResult &r, std::vector<int32_t> &out);
 // waves per team (a multiple of kBlockWaves) of a batched / served query of this kind
 // and size, and its instance class (CLS_*)
 int team_waves(ua_ctx *c, int kind, int64_t cells);

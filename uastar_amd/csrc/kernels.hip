@@ -140,18 +140,26 @@ __global__ void k_grid_mark_blocked(uint64_t *best, const uint64_t *bm, int H, i
     if (!((bm[(size_t)x * wpr + (y >> 6)] >> (y & 63)) & 1ull)) best[i] = ~kOpenBit;
 }
 
-// Byte-per-cell input (pathway.hpp:116-119, 0xFF open / 0 blocked) to the bitmap.
-__global__ void k_grid_pack(const uint8_t *bytes, uint64_t *bm, int H, int W, int wpr) {
+// Byte-per-cell input (pathway.hpp:116-119) to the bitmap. Each byte is a cell's mask of
+// allowed out-moves (bit d = move d, pathway/CPU-solver.cpp:64-66); the bitmap holds
+// all-or-nothing masks only, 0xFF open and 0 blocked, which is what every input module
+// writes (input/custom.cpp:23, input/zigzag.cpp:26-39). Any other byte sets *bad, and
+// the caller rejects the grid.
+__global__ void k_grid_pack(const uint8_t *bytes, uint64_t *bm, int H, int W, int wpr, uint32_t *bad) {
     size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= (size_t)H * wpr) return;
     int x = (int)(w / wpr), y0 = (int)(w % wpr) * 64;
     uint64_t word = 0;
+    bool partial = false;
     for (int b = 0; b < 64; ++b) {
         int y = y0 + b;
         if (y >= W) break;
-        word |= (uint64_t)(bytes[(size_t)x * W + y] != 0) << b;
+        const uint8_t v = bytes[(size_t)x * W + y];
+        partial |= v != 0 && v != 0xFF;
+        word |= (uint64_t)(v != 0) << b;
     }
     bm[w] = word;
+    if (partial) atomicOr(bad, 1u);
 }
 
 // ------------------------------------------------------------------ PDB build (device BFS)
@@ -271,9 +279,10 @@ hipError_t launch_grid_gen(uint64_t *bm, int H, int W, int wpr, uint64_t seed, u
     return hipGetLastError();
 }
 
-hipError_t launch_grid_pack(const uint8_t *bytes, uint64_t *bm, int H, int W, int wpr, hipStream_t st) {
+hipError_t launch_grid_pack(const uint8_t *bytes, uint64_t *bm, int H, int W, int wpr, uint32_t *bad,
+                            hipStream_t st) {
     size_t n = (size_t)H * wpr;
-    hipLaunchKernelGGL(k_grid_pack, dim3(grid_blocks(n, 256)), dim3(256), 0, st, bytes, bm, H, W, wpr);
+    hipLaunchKernelGGL(k_grid_pack, dim3(grid_blocks(n, 256)), dim3(256), 0, st, bytes, bm, H, W, wpr, bad);
     return hipGetLastError();
 }
 

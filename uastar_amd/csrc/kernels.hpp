@@ -7,7 +7,10 @@ namespace ua {
 
 hipError_t launch_grid_gen(uint64_t *bm, int H, int W, int wpr, uint64_t seed, uint64_t thr, int sx,
                            int sy, int ex, int ey, hipStream_t st);
-hipError_t launch_grid_pack(const uint8_t *bytes, uint64_t *bm, int H, int W, int wpr, hipStream_t st);
+// Bytes (0 blocked, 0xFF open) to the bitmap; *bad (zeroed by the caller) becomes nonzero
+// if any byte is a partial out-move mask.
+hipError_t launch_grid_pack(const uint8_t *bytes, uint64_t *bm, int H, int W, int wpr, uint32_t *bad,
+                            hipStream_t st);
 // After a fill of a grid best table with all ones: clears the open bit of blocked cells.
 hipError_t launch_grid_mark_blocked(uint64_t *best, const uint64_t *bm, int H, int W, int wpr, hipStream_t st);
 // The search kernel (k_teams): `blocks` blocks of kBlockWaves wavefronts, cooperative,

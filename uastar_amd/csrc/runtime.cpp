@@ -182,6 +182,7 @@ void fill_result(const QueryDesc &q, const QueryResult &r, ua_result *o) {
     o->expanded = r.expanded;
     o->generated = r.generated;
     o->inserted = r.inserted;
+    o->atomics = r.atomics;
     o->device_ms = ticks_ms(r.t_start, r.t_end);
     o->weight = 1 << q.wshift;
     o->optimal = o->found && q.wshift == 0 && !(r.flags & FLAG_PRUNED);
@@ -326,12 +327,26 @@ extern "C" int ua_grid_from_bytes(ua_ctx *c, int H, int W, const uint8_t *bytes,
     if (!bytes) return fail(c, UA_ERR_INVALID, "null bytes");
     int rc = ua_grid_create(c, H, W, nullptr, out);
     if (rc) return rc;
-    DevBuf tmp;
-    HIPCHK(c, tmp.ensure((size_t)H * W));
-    HIPCHK(c, hipMemcpyAsync(tmp.p, bytes, (size_t)H * W, hipMemcpyHostToDevice, c->st));
-    HIPCHK(c, launch_grid_pack(tmp.as<uint8_t>(), (*out)->bm, H, W, (*out)->wpr, c->st));
-    HIPCHK(c, hipStreamSynchronize(c->st));
+    const size_t n = (size_t)H * W, flag_off = (n + 255) & ~(size_t)255;
+    DevBuf tmp;  // the bytes, then the partial-mask flag
+    uint32_t bad = 0;
+    hipError_t e = tmp.ensure(flag_off + 4);
+    if (!e) e = hipMemcpyAsync(tmp.p, bytes, n, hipMemcpyHostToDevice, c->st);
+    if (!e) e = hipMemsetAsync(tmp.as<char>() + flag_off, 0, 4, c->st);
+    if (!e) e = launch_grid_pack(tmp.as<uint8_t>(), (*out)->bm, H, W, (*out)->wpr,
+                                 reinterpret_cast<uint32_t *>(tmp.as<char>() + flag_off), c->st);
+    if (!e) e = hipMemcpyAsync(&bad, tmp.as<char>() + flag_off, 4, hipMemcpyDeviceToHost, c->st);
+    if (!e) e = hipStreamSynchronize(c->st);
     tmp.release();
+    if (e || bad) {
+        ua_grid_free(c, *out);
+        *out = nullptr;
+        if (e) return fail(c, e == hipErrorOutOfMemory ? UA_ERR_OOM : UA_ERR_HIP, "grid upload: %s",
+                           hipGetErrorString(e));
+        return fail(c, UA_ERR_INVALID,
+                    "cell bytes must be 0 (blocked) or 0xFF (open): a partial out-move mask cannot be held "
+                    "by the 1-bit grid");
+    }
     return UA_OK;
 }
 
@@ -379,28 +394,64 @@ extern "C" int ua_grid_solve(ua_ctx *c, const ua_grid *g, int sx, int sy, int ex
     return res->status;
 }
 
+namespace uart {
+
+// The path of a finished search: start..goal cell ids (grid) or blank moves (puzzle),
+// walked on the device from the goal's parent directions. A found answer fixes the
+// length (orth + diag steps, or the move count), so the walk's buffer is that long.
+int fetch_path(ua_ctx *c, const QueryDesc &q, const QueryResult &r, std::vector<int32_t> &out) {
+    out.clear();
+    if (!r.found || r.status != ST_OK) return fail(c, UA_ERR_INVALID, "query has no path");
+    int64_t want;
+    if (q.kind == KIND_GRID) {
+        int64_t a, b;
+        key_to_pair(r.goal, &a, &b);
+        want = a + b + 1;
+    } else {
+        if (r.rounds == 0) return UA_OK;  // start == goal
+        want = (int64_t)(r.goal >> 32);
+    }
+    if (want < 1 || want > (int64_t)q.H * q.W + 1 + (q.kind == KIND_GRID ? 0 : 1 << 20))
+        return fail(c, UA_ERR_HIP, "implausible path length %lld", (long long)want);
+    const size_t len_off = align256((size_t)want * 4);
+    HIPCHK(c, c->pathbuf.ensure(len_off + 64));
+    int64_t *d_len = reinterpret_cast<int64_t *>(c->pathbuf.as<char>() + len_off);
+    if (q.kind == KIND_GRID)
+        HIPCHK(c, launch_grid_path(q.best, q.H, q.W, q.sx, q.sy, q.ex, q.ey, c->pathbuf.as<int32_t>(), want, d_len,
+                                   c->st));
+    else
+        HIPCHK(c, launch_puzzle_moves(q.n, q.best, q.vals, q.hash_log2, r.goal & 0xffffffffull,
+                                      c->pathbuf.as<int32_t>(), want, d_len, c->st));
+    int64_t n = 0;
+    HIPCHK(c, hipMemcpyAsync(&n, d_len, 8, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    if (n != want) return fail(c, UA_ERR_HIP, "broken parent chain (%lld of %lld)", (long long)n, (long long)want);
+    std::vector<int32_t> tmp(n);
+    HIPCHK(c, hipMemcpyAsync(tmp.data(), c->pathbuf.p, (size_t)n * 4, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    out.assign(tmp.rbegin(), tmp.rend());  // the walk ran goal -> start
+    return UA_OK;
+}
+
+}  // namespace uart
+
+// A path into a caller buffer: *len gets its length; with ids == nullptr only the length.
+static int copy_path(ua_ctx *c, const std::vector<int32_t> &p, int32_t *ids, int64_t cap, int64_t *len) {
+    *len = (int64_t)p.size();
+    if (!ids) return UA_OK;
+    if ((int64_t)p.size() > cap) return fail(c, UA_ERR_INVALID, "path needs %lld entries", (long long)p.size());
+    std::copy(p.begin(), p.end(), ids);
+    return UA_OK;
+}
+
 extern "C" int ua_grid_path(ua_ctx *c, int32_t *ids, int64_t cap, int64_t *len) {
     if (!c || !len) return fail(c, UA_ERR_INVALID, "null argument");
     if (!c->last_valid || c->last_q.kind != KIND_GRID || !c->last_r.found)
         return fail(c, UA_ERR_INVALID, "no solved grid query");
-    const QueryDesc &q = c->last_q;
-    int64_t maxlen = (int64_t)q.H * q.W;
-    size_t len_off = ((size_t)maxlen * 4 + 255) & ~(size_t)255;
-    HIPCHK(c, c->pathbuf.ensure(len_off + 64));
-    int64_t *d_len = reinterpret_cast<int64_t *>(c->pathbuf.as<char>() + len_off);
-    HIPCHK(c, launch_grid_path(q.best, q.H, q.W, q.sx, q.sy, q.ex, q.ey, c->pathbuf.as<int32_t>(), maxlen,
-                               d_len, c->st));
-    int64_t n = 0;
-    HIPCHK(c, hipMemcpyAsync(&n, d_len, 8, hipMemcpyDeviceToHost, c->st));
-    HIPCHK(c, hipStreamSynchronize(c->st));
-    if (n < 0) return fail(c, UA_ERR_HIP, "broken parent chain");
-    *len = n;
-    if (!ids) return UA_OK;  // size query
-    if (n > cap) return fail(c, UA_ERR_INVALID, "path needs %lld entries", (long long)n);
-    std::vector<int32_t> tmp(n);
-    HIPCHK(c, hipMemcpy(tmp.data(), c->pathbuf.p, (size_t)n * 4, hipMemcpyDeviceToHost));
-    std::reverse_copy(tmp.begin(), tmp.end(), ids);
-    return UA_OK;
+    std::vector<int32_t> p;
+    int rc = fetch_path(c, c->last_q, c->last_r, p);
+    if (rc) return rc;
+    return copy_path(c, p, ids, cap, len);
 }
 
 extern "C" int ua_puzzle_build_pdb(ua_ctx *c, int n) {
@@ -506,27 +557,10 @@ extern "C" int ua_puzzle_moves(ua_ctx *c, int32_t *dirs, int64_t cap, int64_t *l
     if (!c || !len) return fail(c, UA_ERR_INVALID, "null argument");
     if (!c->last_valid || c->last_q.kind != KIND_PUZZLE || !c->last_r.found)
         return fail(c, UA_ERR_INVALID, "no solved puzzle query");
-    const QueryDesc &q = c->last_q;
-    if (q.start_state == 0 || c->last_r.rounds == 0) {  // start == goal
-        *len = 0;
-        return UA_OK;
-    }
-    const int64_t maxlen = 4096;
-    HIPCHK(c, c->pathbuf.ensure(maxlen * 4 + 64));
-    int64_t *d_len = reinterpret_cast<int64_t *>(c->pathbuf.as<char>() + maxlen * 4);
-    HIPCHK(c, launch_puzzle_moves(q.n, q.best, q.vals, q.hash_log2, c->last_r.goal & 0xffffffffull,
-                                  c->pathbuf.as<int32_t>(), maxlen, d_len, c->st));
-    int64_t n = 0;
-    HIPCHK(c, hipMemcpyAsync(&n, d_len, 8, hipMemcpyDeviceToHost, c->st));
-    HIPCHK(c, hipStreamSynchronize(c->st));
-    if (n < 0) return fail(c, UA_ERR_HIP, "broken parent chain");
-    *len = n;
-    if (!dirs) return UA_OK;  // size query
-    if (n > cap) return fail(c, UA_ERR_INVALID, "moves need %lld entries", (long long)n);
-    std::vector<int32_t> tmp(n);
-    HIPCHK(c, hipMemcpy(tmp.data(), c->pathbuf.p, (size_t)n * 4, hipMemcpyDeviceToHost));
-    std::reverse_copy(tmp.begin(), tmp.end(), dirs);
-    return UA_OK;
+    std::vector<int32_t> p;
+    int rc = fetch_path(c, c->last_q, c->last_r, p);
+    if (rc) return rc;
+    return copy_path(c, p, dirs, cap, len);
 }
 
 // ------------------------------------------------------------------ batch
@@ -797,18 +831,10 @@ extern "C" int ua_batch_solve(ua_ctx *c, const ua_query *qs, int64_t nq, ua_resu
 extern "C" int ua_batch_path(ua_ctx *c, int64_t i, int32_t *outp, int64_t cap, int64_t *len) {
     if (!c || !len || i < 0 || i >= (int64_t)c->b_desc.size()) return fail(c, UA_ERR_INVALID, "bad query index");
     if (c->b_arena_of[i] < 0) return fail(c, UA_ERR_INVALID, "query %lld: search state no longer resident", (long long)i);
-    // temporarily expose query i as "last" and reuse the plain path walkers
-    QueryDesc sq = c->last_q;
-    QueryResult sr = c->last_r;
-    bool sv = c->last_valid;
-    c->last_q = c->b_desc[i];
-    c->last_r = c->b_res[i];
-    c->last_valid = true;
-    int rc = c->last_q.kind == KIND_GRID ? ua_grid_path(c, outp, cap, len) : ua_puzzle_moves(c, outp, cap, len);
-    c->last_q = sq;
-    c->last_r = sr;
-    c->last_valid = sv;
-    return rc;
+    std::vector<int32_t> p;
+    int rc = fetch_path(c, c->b_desc[i], c->b_res[i], p);
+    if (rc) return rc;
+    return copy_path(c, p, outp, cap, len);
 }
 
 // ------------------------------------------------------------------ diagnostics

@@ -40,7 +40,7 @@ struct TeamMem {
     RoundSlot *view;   // LDS: the round slot as read by one lane, for the whole block
     uint64_t *seg;     // [2][segcap]
     int segcap;
-    uint64_t *stats;   // [4]: pops, expanded, generated, inserted
+    uint64_t *stats;   // [5]: pops, expanded, generated, inserted, atomics
 };
 
 // Relaxed atomics at a scope chosen at run time. A one-block team is the only user of
@@ -133,6 +133,16 @@ struct Scope {
 };
 
 __device__ __forceinline__ int64_t now_ticks() { return (int64_t)__builtin_amdgcn_s_memrealtime(); }
+
+// Per-lane search counters of one launch (32-bit: a lane's counts stay below 2^32 for any
+// round limit), summed over the team at the end of a query:
+//   exp  states expanded (pops that were not stale)
+//   gen  legal moves generated
+//   ins  successors whose cost the lane's atomic lowered (inserted or improved)
+//   atom device-memory atomics issued on the search state (relaxing min, hash-claiming CAS)
+struct Ctr {
+    uint32_t exp = 0, gen = 0, ins = 0, atom = 0;
+};
 
 // A group of `bpg` consecutive blocks of a cooperative launch. Round state and the
 // insert buffer live in global memory and cross a generation barrier over the
@@ -256,8 +266,8 @@ struct GridProb {
     // *gk_out receives the cell's g key when the cell was expanded.
     template <class SC>
     __device__ __forceinline__ void expand(const SC &sc, uint64_t it, uint64_t (&s)[MAXS], int &cnt,
-                                           uint64_t &goalmin, uint32_t &err, int64_t &n_exp, int64_t &n_gen,
-                                           int64_t &n_ins, uint64_t *gk_out = nullptr) const {
+                                           uint64_t &goalmin, uint32_t &err, Ctr &k,
+                                           uint64_t *gk_out = nullptr) const {
         const uint64_t cell = it & cmask;
         const int x = (int)(cell >> q.ybits), y = (int)(cell & ymask);
         if (x >= q.H || y >= q.W) {  // never produced by a correct round: fail the query, touch nothing
@@ -268,10 +278,10 @@ struct GridProb {
         const uint64_t bw = sc.load_weak(bp);
         const uint64_t gk = bw >> kGridKeyShift;
         if (((gk + oct(uabs(x - q.ex), uabs(y - q.ey))) >> q.qshift) != (it >> q.cbits)) return;  // stale
-        ++n_exp;
+        ++k.exp;
         if (!(bw & kOpenBit)) return;  // source-cell mask rule: a blocked cell has no moves
         if (gk_out) *gk_out = gk;
-        relax(sc, x, y, gk, bp, s, cnt, goalmin, err, n_gen, n_ins);
+        relax(sc, x, y, gk, bp, s, cnt, goalmin, err, k);
     }
 
     // Continuation inside the round: when the only successors a lane inserted are a child
@@ -297,21 +307,19 @@ struct GridProb {
     }
     template <class SC>
     __device__ __forceinline__ void expand_child(const SC &sc, uint64_t child, uint64_t gk, uint64_t (&s)[MAXS],
-                                                 int &cnt, uint64_t &goalmin, uint32_t &err, int64_t &n_exp,
-                                                 int64_t &n_gen, int64_t &n_ins) const {
+                                                 int &cnt, uint64_t &goalmin, uint32_t &err, Ctr &k) const {
         const uint64_t cell = child & cmask;
         const int x = (int)(cell >> q.ybits), y = (int)(cell & ymask);
         UA_GLOBAL uint64_t *const bp = gptr(q.best) + ((uint64_t)(uint32_t)x * (uint32_t)q.W + (uint32_t)y);
-        ++n_exp;
-        relax(sc, x, y, gk, bp, s, cnt, goalmin, err, n_gen, n_ins);
+        ++k.exp;
+        relax(sc, x, y, gk, bp, s, cnt, goalmin, err, k);
     }
 
     // Relaxes the moves out of open cell (x, y) of g key gk (bp its best word).
     template <class SC>
     __device__ __forceinline__ void relax(const SC &sc, int x, int y, uint64_t gk, UA_GLOBAL uint64_t *bp,
                                           uint64_t (&s)[MAXS],
-                                          int &cnt, uint64_t &goalmin, uint32_t &err, int64_t &n_gen,
-                                          int64_t &n_ins) const {
+                                          int &cnt, uint64_t &goalmin, uint32_t &err, Ctr &k) const {
         const int gx = x - q.ex, gy = y - q.ey;  // offsets to the goal
         const int64_t Wl = q.W;
         uint64_t w[8];
@@ -339,7 +347,7 @@ struct GridProb {
                 if (!((live >> d) & 1u)) continue;
                 const bool is_goal = gx + kDX[d] == 0 && gy + kDY[d] == 0;
                 if (!is_goal && !(w[d] & kOpenBit)) continue;
-                ++n_gen;
+                ++k.gen;
                 const uint64_t ng = gk + (d < 4 ? KEY_ONE : KEY_SQRT2);
                 const uint64_t nf = ng + oct(uabs(gx + kDX[d]), uabs(gy + kDY[d]));
                 if (nf >= KEY_LIMIT) {
@@ -347,6 +355,7 @@ struct GridProb {
                     continue;
                 }
                 if ((w[d] >> kGridKeyShift) <= ng) continue;
+                ++k.atom;
                 old[j] = sc.fmin(bp + (kDX[d] * Wl + kDY[d]), (ng << kGridKeyShift) | (w[d] & kOpenBit) | (uint64_t)d);
                 if (!is_goal)
                     s[d] = ((nf >> q.qshift) << q.cbits) | ((uint64_t)(uint32_t)(x + kDX[d]) << q.ybits) |
@@ -360,7 +369,7 @@ struct GridProb {
                     s[d] = SENT;
                     continue;
                 }
-                ++n_ins;
+                ++k.ins;
                 if (gx + kDX[d] == 0 && gy + kDY[d] == 0) goalmin = goalmin < ng ? goalmin : ng;
                 else ++cnt;
             }
@@ -506,13 +515,15 @@ struct PuzzleProb {
     }
     __device__ bool trivial() const { return q.start_state == goal_b; }
     template <class SC>
-    __device__ __forceinline__ uint64_t find_or_insert(const SC &sc, uint64_t b, uint32_t &err) const {
+    __device__ __forceinline__ uint64_t find_or_insert(const SC &sc, uint64_t b, uint32_t &err,
+                                                       uint32_t &atom) const {
         uint64_t slot = mix64(b) & hmask;
         for (int i = 0; i < MAXPROBE; ++i) {
             uint64_t k = sc.load_weak(gptr(q.best) + slot);
             if (k == b) return slot;
             if (k == 0) {
                 uint64_t exp = 0;
+                ++atom;
                 if (sc.cas(gptr(q.best) + slot, exp, b) || exp == b) return slot;
             }
             slot = (slot + 1) & hmask;
@@ -521,8 +532,8 @@ struct PuzzleProb {
         return SENT;
     }
     __device__ uint64_t seed(const Scope &sc) const {
-        uint32_t err = 0;
-        uint64_t slot = find_or_insert(sc, q.start_state, err);
+        uint32_t err = 0, atom = 0;
+        uint64_t slot = find_or_insert(sc, q.start_state, err, atom);
         sc.store(&q.vals[slot], (uint32_t)15);
         uint64_t h = (uint64_t)puzzle_h(tb, q.start_state, pdb);
         return (h << 56) | (255ull << 48) | slot;
@@ -533,8 +544,7 @@ struct PuzzleProb {
     }
     template <class SC>
     __device__ __forceinline__ void expand(const SC &sc, uint64_t it, uint64_t (&s)[MAXS], int &cnt,
-                                           uint64_t &goalmin, uint32_t &err, int64_t &n_exp, int64_t &n_gen,
-                                           int64_t &n_ins) const {
+                                           uint64_t &goalmin, uint32_t &err, Ctr &k) const {
         uint64_t slot = it & ((1ull << 48) - 1);
         if (slot > hmask) {  // never produced by a correct round: fail the query, touch nothing
             err |= FLAG_ABORT;
@@ -545,7 +555,7 @@ struct PuzzleProb {
         uint32_t v = sc.load_weak(gptr(q.vals) + slot);
         uint32_t gb = v >> 4;
         if (gb != gi) return;  // stale
-        ++n_exp;
+        ++k.exp;
         const int dprev = (int)(v & 15u);
         const int bl = nib_pos(b, 0);
         const int bx = n == 4 ? bl >> 2 : bl / 3, by = bl - bx * n;
@@ -574,24 +584,27 @@ struct PuzzleProb {
         for (int d = 0; d < 4; ++d) {
             ts[d] = SENT;
             if (!((legal >> d) & 1u)) continue;
-            ++n_gen;
+            ++k.gen;
             const uint32_t f = (ng >> q.wshift) + (uint32_t)h[d];
             if (f > 255u) {
                 err |= q.bounded ? FLAG_PRUNED : FLAG_RANGE;
                 continue;
             }
-            ts[d] = find_or_insert(sc, ns[d], err);
+            ts[d] = find_or_insert(sc, ns[d], err, k.atom);
         }
         uint32_t old[4];
 #pragma unroll
         for (int d = 0; d < 4; ++d) {
             old[d] = 0;  // no atomic: never an improvement
-            if (ts[d] != SENT) old[d] = sc.fmin(gptr(q.vals) + ts[d], (ng << 4) | (uint32_t)d);
+            if (ts[d] != SENT) {
+                ++k.atom;
+                old[d] = sc.fmin(gptr(q.vals) + ts[d], (ng << 4) | (uint32_t)d);
+            }
         }
 #pragma unroll
         for (int d = 0; d < 4; ++d) {
             if (ts[d] == SENT || (old[d] >> 4) <= ng) continue;
-            ++n_ins;
+            ++k.ins;
             if (ns[d] == goal_b) {
                 const uint64_t gk = ((uint64_t)ng << 32) | ts[d];
                 goalmin = goalmin < gk ? goalmin : gk;
@@ -677,12 +690,14 @@ struct PuzzleProb5 {
         return h;
     }
     template <class SC>
-    __device__ __forceinline__ uint64_t find_or_insert(const SC &sc, uint64_t b0, uint64_t b1, uint32_t &err) const {
+    __device__ __forceinline__ uint64_t find_or_insert(const SC &sc, uint64_t b0, uint64_t b1, uint32_t &err,
+                                                       uint32_t &atom) const {
         uint64_t slot = mix64(b0 ^ mix64(b1)) & hmask;
         for (int i = 0; i < MAXPROBE; ++i) {
             uint64_t k0 = sc.load_weak(gptr(q.best) + (2 * slot));
             if (k0 == 0) {
                 uint64_t exp = 0;
+                ++atom;
                 if (sc.cas(gptr(q.best) + (2 * slot), exp, b0)) {
                     sc.store(gptr(q.best) + (2 * slot + 1), b1);
                     return slot;
@@ -704,8 +719,8 @@ struct PuzzleProb5 {
         return SENT;
     }
     __device__ uint64_t seed(const Scope &sc) const {
-        uint32_t err = 0;
-        uint64_t slot = find_or_insert(sc, q.start_state, q.start_state2, err);
+        uint32_t err = 0, atom = 0;
+        uint64_t slot = find_or_insert(sc, q.start_state, q.start_state2, err, atom);
         if (slot == SENT) return SENT;
         sc.store(gptr(q.vals) + (slot), (uint32_t)15);
         uint64_t h = (uint64_t)h_full(q.start_state, q.start_state2);
@@ -717,8 +732,7 @@ struct PuzzleProb5 {
     }
     template <class SC>
     __device__ __forceinline__ void expand(const SC &sc, uint64_t it, uint64_t (&s)[MAXS], int &cnt,
-                                           uint64_t &goalmin, uint32_t &err, int64_t &n_exp, int64_t &n_gen,
-                                           int64_t &n_ins) const {
+                                           uint64_t &goalmin, uint32_t &err, Ctr &k) const {
         uint64_t slot = it & ((1ull << 40) - 1);
         if (slot > hmask) {  // never produced by a correct round: fail the query, touch nothing
             err |= FLAG_ABORT;
@@ -731,7 +745,7 @@ struct PuzzleProb5 {
         uint32_t v = sc.load_weak(gptr(q.vals) + (slot));
         uint32_t gb = v >> 4;
         if (gb != gi) return;  // stale
-        ++n_exp;
+        ++k.exp;
         int dprev = (int)(v & 15u);
         int bl = pos5(w0, w1, 0);
         int bx = bl / 5, by = bl - bx * 5;
@@ -740,7 +754,7 @@ struct PuzzleProb5 {
             if (dprev < 4 && d == (dprev ^ 1)) continue;  // the move back to the parent
             int nx = bx + kDX[d], ny = by + kDY[d];
             if (nx < 0 || nx >= 5 || ny < 0 || ny >= 5) continue;
-            ++n_gen;
+            ++k.gen;
             int nb = nx * 5 + ny;
             uint64_t t = cell5(w0, w1, nb);
             uint64_t n0 = w0, n1 = w1;
@@ -755,11 +769,12 @@ struct PuzzleProb5 {
                 err |= q.bounded ? FLAG_PRUNED : FLAG_RANGE;
                 continue;
             }
-            uint64_t ns_slot = find_or_insert(sc, n0, n1, err);
+            uint64_t ns_slot = find_or_insert(sc, n0, n1, err, k.atom);
             if (ns_slot == SENT) continue;
+            ++k.atom;
             uint32_t old = sc.fmin(gptr(q.vals) + (ns_slot), (ng << 4) | (uint32_t)d);
             if ((old >> 4) <= ng) continue;
-            ++n_ins;
+            ++k.ins;
             if (n0 == g0 && n1 == g1) {
                 uint64_t gk = ((uint64_t)ng << 32) | ns_slot;
                 goalmin = goalmin < gk ? goalmin : gk;
@@ -787,7 +802,7 @@ __device__ __forceinline__ void team_solve(const QueryDesc &q, QueryResult *res,
             res->found = 1;
             res->goal = 0;
             res->rounds = 0;
-            res->pops = res->expanded = res->generated = res->inserted = 0;
+            res->pops = res->expanded = res->generated = res->inserted = res->atomics = 0;
             res->flags = 0;
         }
         return;
@@ -806,6 +821,7 @@ __device__ __forceinline__ void team_solve(const QueryDesc &q, QueryResult *res,
             sc.store(&slots[i].err, 0u);
         }
         for (int i = 0; i < 4; ++i) sc.store(&stats[i], resumed ? q.resume[3 + i] : (uint64_t)0);
+        sc.store(&stats[4], resumed ? q.resume[8] : (uint64_t)0);
     }
     if (resumed) {
         heap.root = rec_heap[l];
@@ -820,7 +836,8 @@ __device__ __forceinline__ void team_solve(const QueryDesc &q, QueryResult *res,
     }
     if (!team.sync([] {})) return;
 
-    int64_t n_pop = 0, n_exp = 0, n_gen = 0, n_ins = 0;
+    uint32_t n_pop = 0;
+    Ctr k;
     uint64_t best_goal = resumed ? q.resume[2] : SENT;
     uint32_t pruned = resumed ? (uint32_t)q.resume[7] & FLAG_PRUNED : 0u;
     int32_t status = ST_OK;
@@ -844,8 +861,8 @@ __device__ __forceinline__ void team_solve(const QueryDesc &q, QueryResult *res,
         uint64_t gk = 0;
         if (it != SENT) {
             ++n_pop;
-            if constexpr (Prob::kContinue) P.expand(sc, it, s, c, goalmin, err, n_exp, n_gen, n_ins, &gk);
-            else P.expand(sc, it, s, c, goalmin, err, n_exp, n_gen, n_ins);
+            if constexpr (Prob::kContinue) P.expand(sc, it, s, c, goalmin, err, k, &gk);
+            else P.expand(sc, it, s, c, goalmin, err, k);
         }
         // appends the lane's keys -- s[] and `keep` unless SENT, c in all -- to the round's
         // insert buffer
@@ -877,8 +894,7 @@ __device__ __forceinline__ void team_solve(const QueryDesc &q, QueryResult *res,
                         s[i] = SENT;
                     }
                     c = keep != SENT;
-                    P.expand_child(sc, child, gk + (dc < 4 ? KEY_ONE : KEY_SQRT2), s, c, goalmin, err, n_exp, n_gen,
-                                   n_ins);
+                    P.expand_child(sc, child, gk + (dc < 4 ? KEY_ONE : KEY_SQRT2), s, c, goalmin, err, k);
                 }
             }
         }
@@ -964,15 +980,14 @@ __device__ __forceinline__ void team_solve(const QueryDesc &q, QueryResult *res,
     }
 
     // ---- statistics and result
-    n_pop = wave_sum64(n_pop);
-    n_exp = wave_sum64(n_exp);
-    n_gen = wave_sum64(n_gen);
-    n_ins = wave_sum64(n_ins);
+    const int64_t t_pop = wave_sum64(n_pop), t_exp = wave_sum64(k.exp), t_gen = wave_sum64(k.gen),
+                  t_ins = wave_sum64(k.ins), t_atom = wave_sum64(k.atom);
     if (l == 0) {
-        sc.fadd(&stats[0], (uint64_t)n_pop);
-        sc.fadd(&stats[1], (uint64_t)n_exp);
-        sc.fadd(&stats[2], (uint64_t)n_gen);
-        sc.fadd(&stats[3], (uint64_t)n_ins);
+        sc.fadd(&stats[0], (uint64_t)t_pop);
+        sc.fadd(&stats[1], (uint64_t)t_exp);
+        sc.fadd(&stats[2], (uint64_t)t_gen);
+        sc.fadd(&stats[3], (uint64_t)t_ins);
+        sc.fadd(&stats[4], (uint64_t)t_atom);
     }
     team.sync([] {});
     if (team.leader()) {
@@ -981,6 +996,7 @@ __device__ __forceinline__ void team_solve(const QueryDesc &q, QueryResult *res,
             q.resume[2] = best_goal;
             for (int i = 0; i < 4; ++i) q.resume[3 + i] = sc.load(&stats[i]);
             q.resume[7] = pruned;
+            q.resume[8] = sc.load(&stats[4]);
             q.resume[0] = 1;
         }
         res->status = status;
@@ -991,6 +1007,7 @@ __device__ __forceinline__ void team_solve(const QueryDesc &q, QueryResult *res,
         res->expanded = (int64_t)sc.load(&stats[1]);
         res->generated = (int64_t)sc.load(&stats[2]);
         res->inserted = (int64_t)sc.load(&stats[3]) + 1;
+        res->atomics = (int64_t)sc.load(&stats[4]);
         res->flags = pruned;
     }
 }

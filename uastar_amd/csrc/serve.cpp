@@ -93,6 +93,7 @@ struct ServeEngine {
     std::deque<Job> pending;
     std::vector<Job> active;
     std::vector<std::pair<uint64_t, ua_result>> done;
+    std::map<uint64_t, std::vector<int32_t>> paths;  // UA_FLAG_SERVE_PATHS: kept until fetched
     uint64_t next_ticket = 1;
     Pool pool;
     bool pool_ready = false;
@@ -345,11 +346,14 @@ extern "C" int ua_serve_step(ua_ctx *c, int64_t *n_active, int64_t *n_pending) {
                 still.push_back(j);
                 continue;
             }
-            e->pool.release(j.off, j.bytes);
-            if (j.bytes2) e->pool.release(j.off2, j.bytes2);
-            j.bytes2 = 0;
-            j.wide = false;
+            auto give_back = [&] {
+                e->pool.release(j.off, j.bytes);
+                if (j.bytes2) e->pool.release(j.off2, j.bytes2);
+                j.bytes2 = 0;
+                j.wide = false;
+            };
             if (r.status == ST_CAPACITY && j.scale < 256) {  // restart with a larger plan
+                give_back();
                 j.scale *= 4;
                 j.rounds = 0;
                 std::lock_guard<std::mutex> g(e->mu);
@@ -358,6 +362,19 @@ extern "C" int ua_serve_step(ua_ctx *c, int64_t *n_active, int64_t *n_pending) {
             }
             ua_result out;
             fill_result(j.d, r, &out);
+            if ((c->cfg.flags & UA_FLAG_SERVE_PATHS) && out.status == UA_OK) {
+                // the walk reads the query's table, which goes back to the pool below
+                std::vector<int32_t> path;
+                const int prc = fetch_path(c, j.d, r, path);
+                if (prc) {  // the query fails with the walk's error (ua_last_error has it)
+                    out.status = prc;
+                    out.found = 0;
+                } else {
+                    std::lock_guard<std::mutex> g(e->mu);
+                    e->paths[j.ticket] = std::move(path);
+                }
+            }
+            give_back();
             out.kernel_ms = j.kernel_ms;
             out.device_ms = j.device_ms;
             out.latency_ms = std::chrono::duration<double, std::milli>(now - j.submit).count();
@@ -397,6 +414,22 @@ extern "C" int ua_serve_poll(ua_ctx *c, uint64_t *tickets, ua_result *results, i
     }
     e->done.erase(e->done.begin(), e->done.begin() + n);
     *n_out = n;
+    return UA_OK;
+}
+
+extern "C" int ua_serve_path(ua_ctx *c, uint64_t ticket, int32_t *out, int64_t cap, int64_t *len) {
+    if (!c || !len) return fail(c, UA_ERR_INVALID, "null argument");
+    if (!(c->cfg.flags & UA_FLAG_SERVE_PATHS))
+        return fail(c, UA_ERR_INVALID, "served paths are kept only with UA_FLAG_SERVE_PATHS");
+    ServeEngine *e = engine(c);
+    std::lock_guard<std::mutex> g(e->mu);
+    auto it = e->paths.find(ticket);
+    if (it == e->paths.end()) return fail(c, UA_ERR_INVALID, "no path kept for ticket %llu", (unsigned long long)ticket);
+    *len = (int64_t)it->second.size();
+    if (!out) return UA_OK;
+    if (*len > cap) return fail(c, UA_ERR_INVALID, "path needs %lld entries", (long long)*len);
+    std::copy(it->second.begin(), it->second.end(), out);
+    e->paths.erase(it);
     return UA_OK;
 }
 
