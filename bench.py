@@ -345,7 +345,7 @@ def cpu_baseline_mixed(ctx, specs, budget_s=25.0, threads=None, seed=11, puzzle_
 
 def run_serve_mixed(args):
     import uastar_amd as U
-    ctx = U.Context(0, graphs=args.graph)
+    ctx = U.Context(0, graphs=args.graph, grid_heaps=args.grid_heaps)
     n_warm, n_timed = args.warmup * QUERIES_PER_STEP, args.steps * QUERIES_PER_STEP
     specs = mixed_stream(n_warm + n_timed)
     materialize(ctx, specs)
@@ -414,6 +414,7 @@ def run_serve_mixed(args):
                                "load x plain capacity",
                    "queries": len(specs), "queries_per_step": QUERIES_PER_STEP,
                    "mode": "serve (continuous batching; team width per class from the autotuner)",
+                   "grid_open_list": "per-wave heaps" if args.grid_heaps else "bucket ring",
                    "load": args.load,
                    "arrival_rate_qps": rate},
         "p50_ms": pct(slat, 50),
@@ -487,7 +488,7 @@ def run_deep_grid(args):
     query), `steps` times after `warmup` small grids. Its rounds are set by the path's
     depth, so the time per round is the figure that bounds the serving stream's tail."""
     import uastar_amd as U
-    ctx = U.Context(0)
+    ctx = U.Context(0, grid_heaps=args.grid_heaps)
     specs = [s for s in mixed_stream(204) if s["kind"] == "grid"]
     deep = max(specs, key=lambda s: max(abs(s["s"][0] - s["g"][0]), abs(s["s"][1] - s["g"][1])))
     warm = [s for s in specs if s["H"] < 1000][: args.warmup]
@@ -548,6 +549,10 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default settings)")
     ap.add_argument("--graph", action="store_true",
                     help="replay each search launch from a hipGraph instead of issuing it call by call")
+    ap.add_argument("--plain-heaps", type=int, default=0,
+                    help="heaps (waves) of the plain whole-GPU solve of the plain workloads (0 = default)")
+    ap.add_argument("--grid-heaps", action="store_true",
+                    help="grids keep one sorted heap per wave instead of the team's bucket ring (comparison)")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (rocprofv3 counter calibration)")
     args = ap.parse_args()
@@ -569,7 +574,7 @@ def main():
     if args.workload == "deep-grid":
         print(json.dumps(run_deep_grid(args)))
         return
-    ctx = U.Context(0, graphs=args.graph)
+    ctx = U.Context(0, graphs=args.graph, grid_heaps=args.grid_heaps, plain_heaps=args.plain_heaps)
 
     big = args.workload == "plain-grid16384"
     specs = grid16384_specs(args.steps) if big else grid1000_specs(args.steps)
@@ -598,7 +603,8 @@ def main():
                                 "start (0,0), goal (16383,16383), one query per whole-GPU launch") if big else
                                ("plain GA*: 1000x1000 grids, Bernoulli(0.30) obstacles from seeds 1.., "
                                 "start (0,0), goal (999,999), one query per whole-GPU launch"),
-                   "mode": "plain", "heaps": ctx.get_tuning()["plain_grid_heaps"]},
+                   "mode": "plain", "heaps": ctx.get_tuning()["plain_grid_heaps"],
+                   "open_list": "per-wave heaps" if args.grid_heaps else "bucket ring"},
         "rounds_p50": pct([r["rounds"] for r in res], 50),
         "answers": [(r["orth"], r["diag"]) for r in res[:4]],
         "p50_ms": pct(lat, 50),

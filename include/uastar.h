@@ -43,7 +43,8 @@ typedef struct ua_grid ua_grid; /* a device-resident grid bitmap */
 
 typedef struct {
     int32_t team_waves;        /* waves (= heaps) per grid query in batch/serve (multiple of 16); 0 = tuned */
-    int32_t plain_heaps;       /* heaps (= waves) of the plain whole-GPU solve; 0 = tuned (untuned: 256 for
+    int32_t plain_heaps;       /* heaps (= waves) of the plain whole-GPU solve; 0 = tuned (untuned: 16 = one
+                                  block on the bucket ring for grids below 2^22 cells, 256 for larger
                                   grids, 4 per CU for puzzles) */
     int32_t max_rounds;        /* 0 = 1<<26 */
     int32_t hash_log2;         /* puzzle hash table size per query; 0 = tuned */
@@ -58,6 +59,9 @@ typedef struct {
 /* ua_config.flags: a served query that finds its goal keeps its path (grid cell ids) or
  * moves (puzzle) on the host until ua_serve_path fetches them */
 #define UA_FLAG_SERVE_PATHS 2
+/* ua_config.flags: grids on one-block teams keep one sorted heap per wave instead of the
+ * team's bucket ring (for comparisons; multi-block grid teams always use heaps) */
+#define UA_FLAG_GRID_HEAPS 4
 
 typedef struct {
     int32_t status;
@@ -144,7 +148,8 @@ typedef struct {
     uint8_t tiles[25];       /* puzzle board, n*n entries */
 } ua_query;
 int ua_batch_solve(ua_ctx *ctx, const ua_query *qs, int64_t nq, ua_result *out);
-/* path / moves of query i of the last ua_batch_solve (while its state is resident) */
+/* path / moves of query i of the last ua_batch_solve: walked on demand while its state is
+ * resident, and walked before its state is freed when a later sub-batch needs the memory */
 int ua_batch_path(ua_ctx *ctx, int64_t i, int32_t *out, int64_t cap, int64_t *len);
 
 /* ---- serving: continuous batching (no reference counterpart). Submitted queries are

@@ -1,5 +1,6 @@
 // emu_batch.cpp -- runs the device batch loop (ua::group_block from search.hpp) for one
-// one-block team of 4, 8 or 16 waves on host threads. Built by tests/test_emu_search.py.
+// one-block team of 4, 8 or 16 waves on host threads, for each kernel set. Built by
+// tests/test_emu_search.py.
 #include "hip_emu.hpp"
 
 #include <vector>
@@ -35,6 +36,10 @@ static int run_batch(const ua::QueryDesc *qs, ua::QueryResult *rs, int nq, int w
 
 extern "C" int emu_batch(const ua::QueryDesc *qs, ua::QueryResult *rs, int nq, int waves) {
     return run_batch<ua::SET_MAIN>(qs, rs, nq, waves);
+}
+// grids on per-wave heaps (k_teams<., SET_GHEAP>)
+extern "C" int emu_batch_gheap(const ua::QueryDesc *qs, ua::QueryResult *rs, int nq, int waves) {
+    return run_batch<ua::SET_GHEAP>(qs, rs, nq, waves);
 }
 // the 5x5 problem set (k_teams<., SET_FIVE>)
 extern "C" int emu_batch5(const ua::QueryDesc *qs, ua::QueryResult *rs, int nq, int waves) {

@@ -25,7 +25,10 @@ class QueryDesc(C.Structure):
                                           "qshift", "heap_cap", "hash_log2", "max_rounds", "heaps")] + [
         ("bitmap", C.c_void_p), ("best", C.c_void_p), ("vals", C.c_void_p), ("heap", C.c_void_p),
         ("start_state", C.c_uint64), ("start_state2", C.c_uint64), ("pdb", C.c_void_p), ("resume", C.c_void_p),
-        ("round_budget", C.c_int32), ("cls", C.c_int32), ("wshift", C.c_int32), ("bounded", C.c_int32)]
+        ("round_budget", C.c_int32), ("cls", C.c_int32), ("wshift", C.c_int32), ("bounded", C.c_int32),
+        ("open_kind", C.c_int32), ("bkt_log2", C.c_int32), ("bshift", C.c_int32), ("bwin", C.c_int32)]
+
+BUCKETS, BUCKET_META_WORDS = 256, 512
 
 RESUME_WORDS_4 = 9 + 4 * 130
 
@@ -46,6 +49,7 @@ def emu():
         os.replace(LIB + ".tmp", LIB)
     L = C.CDLL(LIB)
     L.emu_batch.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int]
+    L.emu_batch_gheap.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int]
     return L
 
 
@@ -57,23 +61,31 @@ def bits_for(v):
 
 
 class Batch:
-    """Host-memory version of the runtime's batch planning (best tables, heaps)."""
+    """Host-memory version of the runtime's batch planning (best tables, bucket rings or
+    heaps). Grids use the team's bucket ring (kernel set SET_MAIN) unless grid_heaps, then
+    one heap per wave (SET_GHEAP)."""
 
-    def __init__(self, waves, max_rounds=100_000):
-        self.waves, self.max_rounds = waves, max_rounds
+    def __init__(self, waves, max_rounds=100_000, grid_heaps=False, bwin=160):
+        self.waves, self.max_rounds, self.grid_heaps, self.bwin = waves, max_rounds, grid_heaps, bwin
         self.descs, self.keep = [], []
 
-    def add_grid(self, graph, H, W, sx, sy, ex, ey, items=65536):
+    def add_grid(self, graph, H, W, sx, sy, ex, ey, items=65536, bkt_log2=None):
         bm = U.bytes_to_bitmap(graph, H, W)
         best = np.full(H * W, np.uint64(~np.uint64(0)))
         best[np.asarray(graph).reshape(-1) == 0] ^= np.uint64(16)  # each word carries its cell's open bit
-        per_heap = (items + 64 * self.waves - 1) // (64 * self.waves)
-        cap = max(per_heap + 1, 4)
-        heap = np.zeros(self.waves * (cap - 1) * 64, np.uint64)
         d = QueryDesc(kind=1, H=H, W=W, sx=sx, sy=sy, ex=ex, ey=ey, wpr=(W + 63) // 64, ybits=bits_for(W))
         d.cbits = bits_for(H) + d.ybits
         d.qshift = max(0, d.cbits - 10)
-        d.heap_cap, d.max_rounds, d.heaps = cap, self.max_rounds, self.waves
+        d.max_rounds, d.heaps = self.max_rounds, self.waves
+        if self.grid_heaps:
+            per_heap = (items + 64 * self.waves - 1) // (64 * self.waves)
+            cap = max(per_heap + 1, 4)
+            heap = np.zeros(self.waves * (cap - 1) * 64, np.uint64)
+            d.heap_cap = cap
+        else:  # as grid_fill_desc: 1/16-step buckets, 1/8 of the capacity each (at least 4096)
+            d.open_kind, d.bshift, d.bwin = 1, max(0, 32 - d.qshift), self.bwin
+            d.bkt_log2 = bkt_log2 if bkt_log2 is not None else bits_for(max(4096, items // 8))
+            heap = np.zeros(BUCKET_META_WORDS + (BUCKETS << d.bkt_log2), np.uint64)
         d.bitmap, d.best, d.heap = bm.ctypes.data, best.ctypes.data, heap.ctypes.data
         self.keep += [bm, best, heap]
         self.descs.append(d)
@@ -96,14 +108,15 @@ class Batch:
     def run(self, lib):
         arr = (QueryDesc * len(self.descs))(*self.descs)
         res = (QueryResult * len(self.descs))()
-        assert lib.emu_batch(arr, res, len(self.descs), self.waves) == 0
+        run = lib.emu_batch_gheap if self.grid_heaps else lib.emu_batch
+        assert run(arr, res, len(self.descs), self.waves) == 0
         return list(res)
 
 
-@pytest.mark.parametrize("waves", [4, 16])
-def test_emulated_batch_grids(emu, oracle, waves):
+@pytest.mark.parametrize("waves,grid_heaps", [(4, False), (16, False), (4, True), (16, True)])
+def test_emulated_batch_grids(emu, oracle, waves, grid_heaps):
     rng = np.random.default_rng(42)
-    b = Batch(waves=waves)
+    b = Batch(waves=waves, grid_heaps=grid_heaps)
     exp = []
     for i in range(6):
         H, W = int(rng.integers(2, 60)), int(rng.integers(2, 60))
@@ -119,6 +132,52 @@ def test_emulated_batch_grids(emu, oracle, waves):
         assert bool(r.found) == e["found"], i
         if e["found"]:
             assert U.key_to_pair(r.goal) == (e["orth"], e["diag"]), i
+
+
+@pytest.mark.parametrize("bwin", [1, 16, 160, 208])
+def test_emulated_bucket_ring_grids(emu, oracle, bwin):
+    """The bucket ring on 40 random grids (2..120 cells a side, obstacle rate 0..0.45,
+    random endpoints) with narrow and wide round windows: costs equal the oracle's, and
+    the expansions stay within bounds of the oracle A*'s (a narrow window pops near-best-
+    first; a wide one takes whatever the round can hold). A query
+    can exceed the oracle where many cells tie at the optimal f (a start whose octile
+    distance is the optimum: every cell on any optimal path has f = C*, and the ring
+    expands the whole tied bucket, the oracle's tie order only some of it)."""
+    rng = np.random.default_rng(77 + bwin)
+    b = Batch(waves=4, bwin=bwin)
+    exp = []
+    for i in range(40):
+        H, W = int(rng.integers(2, 121)), int(rng.integers(2, 121))
+        m = random_grid(rng, H, W, float(rng.uniform(0.0, 0.45)))
+        sx, sy, ex, ey = (int(rng.integers(H)), int(rng.integers(W)), int(rng.integers(H)), int(rng.integers(W)))
+        m[sx, sy] = True
+        g = grid_bytes(m)
+        b.add_grid(g, H, W, sx, sy, ex, ey)
+        exp.append(oracle.grid_astar(g, H, W, sx, sy, ex, ey, "ref"))
+    res = b.run(emu)
+    for i, (r, e) in enumerate(zip(res, exp)):
+        assert r.status in (0, 1), (i, r.status)
+        assert bool(r.found) == e["found"], i
+        if e["found"]:
+            assert U.key_to_pair(r.goal) == (e["orth"], e["diag"]), i
+            assert r.expanded <= 24 * (e["expanded"] + 64), (i, r.expanded, e["expanded"])
+    # a wide window on a small open list pops everything, as the heaps do (which expand
+    # about 8x the oracle's on these grids)
+    total, total_oracle = sum(r.expanded for r in res), sum(e["expanded"] for e in exp)
+    assert total <= (4.0 if bwin > 16 else 1.5) * total_oracle, (total, total_oracle)
+
+
+def test_emulated_bucket_ring_overflow_is_reported(emu, oracle):
+    """A bucket too small for the search's keys fails the query with ST_CAPACITY (the
+    runtime retries it larger) instead of overwriting unconsumed keys."""
+    H = W = 90
+    g = grid_bytes(np.ones((H, W), bool))
+    b = Batch(waves=4)
+    b.add_grid(g, H, W, 0, 45, 89, 45, bkt_log2=2)
+    b.add_grid(g, H, W, 0, 45, 89, 45)
+    res = b.run(emu)
+    assert res[0].status == -3
+    assert res[1].status == 0 and U.key_to_pair(res[1].goal) == (89, 0)
 
 
 def test_emulated_batch_puzzles(emu, oracle):

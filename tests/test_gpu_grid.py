@@ -118,6 +118,31 @@ def test_partial_out_move_masks_are_rejected(ctx):
     assert r["found"] and (r["orth"], r["diag"]) == (61, 8)
 
 
+@pytest.fixture(scope="module")
+def team_ctxs():
+    """Contexts whose plain solves run the other open lists and team shapes: one block on
+    the bucket ring (16 heaps), four blocks on per-wave heaps (64), one block on per-wave
+    heaps (UA_FLAG_GRID_HEAPS)."""
+    cs = {"ring-1-block": U.Context(0, plain_heaps=16), "heaps-4-blocks": U.Context(0, plain_heaps=64),
+          "heaps-1-block": U.Context(0, plain_heaps=16, grid_heaps=True)}
+    yield cs
+    for c in cs.values():
+        c.close()
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_random_small_every_team_shape(team_ctxs, oracle, seed):
+    rng = np.random.default_rng(900 + seed)
+    H, W = int(rng.integers(2, 150)), int(rng.integers(2, 150))
+    m = random_grid(rng, H, W, float(rng.uniform(0.0, 0.45)))
+    sx, sy, ex, ey = int(rng.integers(H)), int(rng.integers(W)), int(rng.integers(H)), int(rng.integers(W))
+    m[sx, sy] = True
+    for name, c in team_ctxs.items():
+        _say(f"random grid seed {seed}: {H}x{W} on {name}")
+        g, ref, res, path = _solve_both(c, oracle, m, sx, sy, ex, ey)
+        _assert_parity(g, H, W, sx, sy, ex, ey, ref, res, path)
+
+
 @pytest.mark.parametrize("seed", range(12))
 def test_random_small(ctx, oracle, seed):
     rng = np.random.default_rng(100 + seed)

@@ -43,6 +43,7 @@ class Config(C.Structure):
 
 FLAG_GRAPH = 1
 FLAG_SERVE_PATHS = 2
+FLAG_GRID_HEAPS = 4
 
 
 class Result(C.Structure):
@@ -201,8 +202,9 @@ class Context:
     """One device context (ua_ctx)."""
 
     def __init__(self, ordinal=0, team_waves=0, plain_heaps=0, max_rounds=0, hash_log2=0, mem_budget=0,
-                 puzzle_team_waves=0, graphs=False, serve_paths=False):
-        flags = (FLAG_GRAPH if graphs else 0) | (FLAG_SERVE_PATHS if serve_paths else 0)
+                 puzzle_team_waves=0, graphs=False, serve_paths=False, grid_heaps=False):
+        flags = ((FLAG_GRAPH if graphs else 0) | (FLAG_SERVE_PATHS if serve_paths else 0) |
+                 (FLAG_GRID_HEAPS if grid_heaps else 0))
         self.cfg = Config(team_waves, plain_heaps, max_rounds, hash_log2, mem_budget, puzzle_team_waves, flags)
         h = C.c_void_p()
         rc = lib().ua_create(ordinal, C.byref(self.cfg), C.byref(h))

@@ -121,13 +121,40 @@ struct QueryDesc {
     // successors that do not fit the hash or the heaps are dropped (FLAG_PRUNED)
     int32_t wshift;
     int32_t bounded;
+    // grids: the open list. OPEN_HEAPS = one batched heap per wave (heap, heap_cap);
+    // OPEN_BUCKETS = a ring of kBuckets buckets shared by a one-block team, bucket
+    // (f_key >> (qshift + bshift)) mod kBuckets, 2^bkt_log2 keys each, stored in `heap`;
+    // a round takes keys from the lowest bwin buckets
+    int32_t open_kind;
+    int32_t bkt_log2;
+    int32_t bshift;
+    int32_t bwin;
+};
+enum : int32_t { OPEN_HEAPS = 0, OPEN_BUCKETS = 1 };
+constexpr int kBuckets = 256;   // buckets of the ring (a power of two)
+constexpr int kBucketMetaWords = 512;  // u64 words before a ring's keys, for its counters (4 KiB)
+
+// One path walk of a finished query (k_paths): grid cell ids or puzzle blank moves,
+// goal..start, at most len entries at offset off of the output; got = the count walked,
+// -1 on a broken chain.
+struct PathJob {
+    const uint64_t *best;    // grid best table / puzzle hash keys
+    const uint32_t *vals;    // puzzle hash values
+    int32_t kind, n, H, W, sx, sy, ex, ey, hash_log2, pad;
+    uint64_t goal_slot;      // puzzle: the goal's hash slot
+    int64_t off, len, got;
 };
 
-// Problem sets of the search kernel: SET_MAIN solves grids and 3x3 / 4x4 puzzles,
-// SET_FIVE 5x5 puzzles. Each set is its own kernel, so that the 5x5 expansion does not
-// raise the register demand of the others (one kernel holding both spills heavily).
-constexpr int SET_MAIN = 0, SET_FIVE = 1;
-UA_HD inline int set_of(const QueryDesc &q) { return q.kind == KIND_PUZZLE && q.n == 5 ? SET_FIVE : SET_MAIN; }
+// Problem sets of the search kernel: SET_MAIN solves grids on bucket rings and 3x3 / 4x4
+// puzzles, SET_FIVE 5x5 puzzles, SET_GHEAP grids on per-wave heaps. Each set is its own
+// kernel, so that no search raises the register demand of another (one kernel holding all
+// of them spills heavily).
+constexpr int SET_MAIN = 0, SET_FIVE = 1, SET_GHEAP = 2;
+constexpr int kSets = 3;
+UA_HD inline int set_of(const QueryDesc &q) {
+    if (q.kind == KIND_PUZZLE) return q.n == 5 ? SET_FIVE : SET_MAIN;
+    return q.open_kind == OPEN_HEAPS ? SET_GHEAP : SET_MAIN;
+}
 
 // The search kernel runs blocks of 16 wavefronts (1024 threads). A team is a group of
 // whole blocks: 16, 32, 64 ... waves (= heaps).

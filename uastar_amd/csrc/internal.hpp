@@ -62,6 +62,7 @@ struct Tuning {
     int grid_waves = 16;          // waves (= heaps) per batched / served grid query below grid_large_cells
     int grid_large_waves = 16;    // ... and at or above grid_large_cells
     int64_t grid_large_cells = 1ll << 22;
+    int grid_bucket_window = 160;  // buckets (1/16 step of f each) a bucket-ring round takes keys from
     int puzzle_waves = 64;        // waves (= heaps) per batched / served puzzle query
     int puzzle_hash_log2[6] = {0, 0, 0, 17, 24, 22};   // batched / served puzzles, by n
     int plain_heaps[2] = {0, 0};  // plain whole-GPU solve, by kind (grid, puzzle); 0 = 4 per CU
@@ -118,6 +119,8 @@ struct ua_ctx {
     std::vector<ua::QueryDesc> b_desc;
     std::vector<ua::QueryResult> b_res;
     std::vector<int64_t> b_order;
+    std::vector<std::vector<int32_t>> b_paths;  // paths walked before their sub-batch's state was released
+    std::vector<char> b_kept;
     // serving
     std::mutex mu;
     ServeEngine *serve = nullptr;
@@ -157,11 +160,14 @@ void fill_result(const QueryDesc &q, const QueryResult &r, ua_result *o);
 // start..goal cell ids (grid) or blank moves (puzzle) of a finished query whose state is
 // still on the device
 int fetch_path(ua_ctx *c, const QueryDesc &q, const QueryResult &r, std::vector<int32_t> &out);
+// the same for n queries in one launch of the walk kernel (out[i] empty: no path)
+int fetch_paths(ua_ctx *c, int n, const QueryDesc *const *qs, const QueryResult *const *rs,
+                std::vector<int32_t> *out);
 // waves per team (a multiple of kBlockWaves) of a batched / served query of this kind
 // and size, and its instance class (CLS_*)
 int team_waves(ua_ctx *c, int kind, int64_t cells);
 int class_of(ua_ctx *c, int kind, int64_t cells);
-int plain_heaps(ua_ctx *c, int kind);
+int plain_heaps(ua_ctx *c, int kind, int64_t cells = 0);
 int batch_hash_log2(ua_ctx *c, int n);
 int plain_hash_log2(ua_ctx *c, int n);
 // One k_teams launch of `blocks` blocks for problem set `set` (step.cpp): the shape, the nq descriptors and

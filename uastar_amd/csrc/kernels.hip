@@ -14,8 +14,8 @@ namespace ua {
 // from a counter. Plain solve: one group of every block and one query. Batch / serve:
 // one-block teams for grids (workgroup scope), multi-block groups for 15-puzzles. MODE
 // only names the caller (0 plain, 1 batch, 2 serving step), so that profiles report
-// the three uses as separate kernels; SET is the problem set (SET_MAIN: grids and 3x3 /
-// 4x4 puzzles, SET_FIVE: 5x5 puzzles). The shape, the counters, the TeamVars and the
+// the three uses as separate kernels; SET is the problem set (SET_MAIN: grids on bucket
+// rings and 3x3 / 4x4 puzzles, SET_FIVE: 5x5 puzzles, SET_GHEAP: grids on per-wave heaps). The shape, the counters, the TeamVars and the
 // queries come from the launch's I/O block (StepHeader), so the arguments of a launch
 // stay the same from step to step; blocks past the shape's own return at once.
 template <int MODE, int SET>
@@ -28,43 +28,37 @@ __global__ __launch_bounds__(1024) void k_teams(char *io, uint64_t *segs) {
 }
 
 // ------------------------------------------------------------------ path walks
-// Single lane: walks the parent directions from the goal (pathway/GPU-kernel.cuh:546-569
-// walks node records instead). ids receive goal..start; *len = count or -1 on a broken chain.
-__global__ void k_grid_path(const uint64_t *best, int H, int W, int sx, int sy, int ex, int ey,
-                            int32_t *ids, int64_t cap, int64_t *len) {
-    if (threadIdx.x || blockIdx.x) return;
+// One lane walks the parent directions from the goal (pathway/GPU-kernel.cuh:546-569
+// walks node records instead); ids receive goal..start. Returns the count, or -1 on a
+// chain that breaks or runs past cap.
+__device__ int64_t walk_grid(const uint64_t *best, int H, int W, int sx, int sy, int ex, int ey, int32_t *ids,
+                             int64_t cap) {
     int x = ex, y = ey;
     int64_t c = 0;
     while (c < cap) {
         ids[c++] = x * W + y;
-        if (x == sx && y == sy) {
-            *len = c;
-            return;
-        }
+        if (x == sx && y == sy) return c;
         int d = (int)(best[(size_t)x * W + y] & 15ull);
         if (d > 7) break;
         x -= kDX[d];
         y -= kDY[d];
         if (x < 0 || x >= H || y < 0 || y >= W) break;
     }
-    *len = -1;
+    return -1;
 }
 
-// Single lane: blank-move indices goal..start (puzzle/GPU-kernel.cuh:462-509 semantics).
+// Blank-move indices goal..start (puzzle/GPU-kernel.cuh:462-509 semantics), for 3x3 / 4x4
+// boards of one u64.
 template <int N>
-__global__ void k_puzzle_moves(const uint64_t *keys, const uint32_t *vals, int hash_log2,
-                               uint64_t goal_slot, int32_t *dirs, int64_t cap, int64_t *len) {
-    if (threadIdx.x || blockIdx.x) return;
+__device__ int64_t walk_moves(const uint64_t *keys, const uint32_t *vals, int hash_log2, uint64_t goal_slot,
+                              int32_t *dirs, int64_t cap) {
     uint64_t hmask = (1ull << hash_log2) - 1;
     uint64_t slot = goal_slot;
     uint64_t b = keys[slot];
     int64_t c = 0;
     while (c <= cap) {
         int d = (int)(vals[slot] & 15u);
-        if (d == 15) {
-            *len = c;
-            return;
-        }
+        if (d == 15) return c;
         if (d > 3 || c == cap) break;
         dirs[c++] = d;
         int bl = nib_pos(b, 0);
@@ -78,23 +72,19 @@ __global__ void k_puzzle_moves(const uint64_t *keys, const uint32_t *vals, int h
         for (; i < PuzzleProb::MAXPROBE && keys[slot] != b; ++i) slot = (slot + 1) & hmask;
         if (i == PuzzleProb::MAXPROBE) break;
     }
-    *len = -1;
+    return -1;
 }
 
 // The same walk over two-word 5x5 boards.
-__global__ void k_puzzle_moves5(const uint64_t *keys, const uint32_t *vals, int hash_log2, uint64_t goal_slot,
-                                int32_t *dirs, int64_t cap, int64_t *len) {
-    if (threadIdx.x || blockIdx.x) return;
+__device__ int64_t walk_moves5(const uint64_t *keys, const uint32_t *vals, int hash_log2, uint64_t goal_slot,
+                               int32_t *dirs, int64_t cap) {
     uint64_t hmask = (1ull << hash_log2) - 1;
     uint64_t slot = goal_slot;
     uint64_t b0 = keys[2 * slot], b1 = keys[2 * slot + 1];
     int64_t c = 0;
     while (c <= cap) {
         int d = (int)(vals[slot] & 15u);
-        if (d == 15) {
-            *len = c;
-            return;
-        }
+        if (d == 15) return c;
         if (d > 3 || c == cap) break;
         dirs[c++] = d;
         int bl = pos5(b0, b1, 0);
@@ -110,7 +100,20 @@ __global__ void k_puzzle_moves5(const uint64_t *keys, const uint32_t *vals, int 
             slot = (slot + 1) & hmask;
         if (i == PuzzleProb5::MAXPROBE) break;
     }
-    *len = -1;
+    return -1;
+}
+
+// Many walks at once, one lane per finished query: job i writes at most jobs[i].len
+// entries at out + jobs[i].off and its count (or -1) to jobs[i].got.
+__global__ void k_paths(PathJob *jobs, int n, int32_t *out) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n) return;
+    PathJob &j = jobs[i];
+    int32_t *o = out + j.off;
+    if (j.kind == KIND_GRID) j.got = walk_grid(j.best, j.H, j.W, j.sx, j.sy, j.ex, j.ey, o, j.len);
+    else if (j.n == 5) j.got = walk_moves5(j.best, j.vals, j.hash_log2, j.goal_slot, o, j.len);
+    else if (j.n == 4) j.got = walk_moves<4>(j.best, j.vals, j.hash_log2, j.goal_slot, o, j.len);
+    else j.got = walk_moves<3>(j.best, j.vals, j.hash_log2, j.goal_slot, o, j.len);
 }
 
 // ------------------------------------------------------------------ grid generator
@@ -300,6 +303,9 @@ const void *teams_fn(int mode, int set) {
     if (set == SET_FIVE)
         return mode == 2 ? (const void *)k_teams<2, SET_FIVE> : mode == 1 ? (const void *)k_teams<1, SET_FIVE>
                                                                           : (const void *)k_teams<0, SET_FIVE>;
+    if (set == SET_GHEAP)
+        return mode == 2 ? (const void *)k_teams<2, SET_GHEAP> : mode == 1 ? (const void *)k_teams<1, SET_GHEAP>
+                                                                           : (const void *)k_teams<0, SET_GHEAP>;
     return mode == 2 ? (const void *)k_teams<2, SET_MAIN> : mode == 1 ? (const void *)k_teams<1, SET_MAIN>
                                                                       : (const void *)k_teams<0, SET_MAIN>;
 }
@@ -309,24 +315,9 @@ hipError_t launch_teams(char *io, int blocks, uint64_t *segs, int mode, int set,
     return hipLaunchCooperativeKernel(teams_fn(mode, set), dim3(blocks), dim3(kBlockWaves * 64), args, 0, st);
 }
 
-hipError_t launch_grid_path(const uint64_t *best, int H, int W, int sx, int sy, int ex, int ey,
-                            int32_t *ids, int64_t cap, int64_t *len, hipStream_t st) {
-    hipLaunchKernelGGL(k_grid_path, dim3(1), dim3(64), 0, st, best, H, W, sx, sy, ex, ey, ids, cap, len);
-    return hipGetLastError();
-}
-
-hipError_t launch_puzzle_moves(int n, const uint64_t *keys, const uint32_t *vals, int hash_log2,
-                               uint64_t goal_slot, int32_t *dirs, int64_t cap, int64_t *len,
-                               hipStream_t st) {
-    if (n == 5)
-        hipLaunchKernelGGL(k_puzzle_moves5, dim3(1), dim3(64), 0, st, keys, vals, hash_log2, goal_slot, dirs, cap,
-                           len);
-    else if (n == 4)
-        hipLaunchKernelGGL(k_puzzle_moves<4>, dim3(1), dim3(64), 0, st, keys, vals, hash_log2, goal_slot,
-                           dirs, cap, len);
-    else
-        hipLaunchKernelGGL(k_puzzle_moves<3>, dim3(1), dim3(64), 0, st, keys, vals, hash_log2, goal_slot,
-                           dirs, cap, len);
+hipError_t launch_paths(PathJob *jobs, int n, int32_t *out, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_paths, dim3(grid_blocks((size_t)n, 64)), dim3(64), 0, st, jobs, n, out);
     return hipGetLastError();
 }
 

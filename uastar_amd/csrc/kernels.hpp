@@ -21,11 +21,8 @@ hipError_t launch_grid_mark_blocked(uint64_t *best, const uint64_t *bm, int H, i
 hipError_t teams_max_blocks(int *per_cu);
 const void *teams_fn(int mode, int set);
 hipError_t launch_teams(char *io, int blocks, uint64_t *segs, int mode, int set, hipStream_t st);
-hipError_t launch_grid_path(const uint64_t *best, int H, int W, int sx, int sy, int ex, int ey,
-                            int32_t *ids, int64_t cap, int64_t *len, hipStream_t st);
-hipError_t launch_puzzle_moves(int n, const uint64_t *keys, const uint32_t *vals, int hash_log2,
-                               uint64_t goal_slot, int32_t *dirs, int64_t cap, int64_t *len,
-                               hipStream_t st);
+// Path walks of finished queries, one lane each (see PathJob).
+hipError_t launch_paths(PathJob *jobs, int n, int32_t *out, hipStream_t st);
 hipError_t build_pdb(int n, uint8_t *pdb, unsigned long long *d_added, hipStream_t st);
 hipError_t launch_heap_selftest(const int *ops, int nops, const uint64_t *keys, uint64_t *storage,
                                 int cap, uint64_t *out, int *status, hipStream_t st);

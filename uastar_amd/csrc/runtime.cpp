@@ -66,10 +66,28 @@ int grid_fill_desc(ua_ctx *c, const ua_grid *g, int sx, int sy, int ex, int ey, 
     q.max_rounds = c->cfg.max_rounds > 0 ? c->cfg.max_rounds : (1 << 26);
     int64_t cells = (int64_t)g->H * g->W;
     int64_t items = std::max<int64_t>(65536, cells / 8) * heap_scale;  // open-list capacity
-    int64_t per_heap = (items + 64ll * heaps - 1) / (64ll * heaps);
-    q.heap_cap = (int)std::min<int64_t>(std::max<int64_t>(per_heap + 1, 4), 1 << 30);
     p = Plan{};
     p.best_bytes = (size_t)cells * 8;
+    if (heaps == kBlockWaves && !(c->cfg.flags & UA_FLAG_GRID_HEAPS)) {
+        // a one-block team: the bucket ring (a ring shared across blocks measured slower
+        // than their heaps: agent-scope counters and two grid barriers a round). Buckets
+        // are 1/16 step of f wide (bshift), a round takes keys from the lowest bwin of them
+        // (at most kBuckets - 47: a child's f exceeds its parent's by at most 2 sqrt(2)
+        // steps, 46 buckets); each holds 1/8 of the open-list capacity, at least 4096 keys.
+        // A grid whose optimum is its octile distance puts every cell of every optimal
+        // path in one bucket (f = C* exactly): the serving stream's deepest grid, 7786^2,
+        // overflowed 1/32. A bucket that fills up fails the query with ST_CAPACITY, which
+        // retries it larger
+        q.open_kind = OPEN_BUCKETS;
+        q.bshift = std::max(0, 32 - q.qshift);
+        q.bwin = std::max(1, std::min(c->tune.grid_bucket_window, kBuckets - 48));
+        q.bkt_log2 = bits_for(std::max<int64_t>(4096, items / 8));
+        q.heap_cap = 0;
+        p.heap_bytes = ((size_t)kBucketMetaWords + ((size_t)kBuckets << q.bkt_log2)) * 8;  // counters, then keys
+        return UA_OK;
+    }
+    int64_t per_heap = (items + 64ll * heaps - 1) / (64ll * heaps);
+    q.heap_cap = (int)std::min<int64_t>(std::max<int64_t>(per_heap + 1, 4), 1 << 30);
     p.heap_bytes = (size_t)heaps * (size_t)(q.heap_cap - 1) * 512;
     return UA_OK;
 }
@@ -193,14 +211,16 @@ void fill_result(const QueryDesc &q, const QueryResult &r, ua_result *o) {
 // Heaps of the plain whole-GPU solve for a query kind (KIND_GRID / KIND_PUZZLE): the
 // configured or tuned count, rounded to whole 4-wave blocks and capped at what one
 // cooperative launch can hold.
-int plain_heaps(ua_ctx *c, int kind) {
+int plain_heaps(ua_ctx *c, int kind, int64_t cells) {
     int per_cu = 0;
     if (teams_max_blocks(&per_cu) != hipSuccess || per_cu < 1) per_cu = 1;
     int tuned = c->tune.plain_heaps[kind == KIND_PUZZLE ? 1 : 0];
-    // untuned defaults: 256 heaps (16 blocks) for a grid, whose search is bound by the
-    // depth of its path (1000^2 and 16384^2 grids solve fastest at 128-256 heaps, against
-    // twice the time at 1024); 4 per CU for a puzzle, whose open list is wide
-    const int dflt = kind == KIND_PUZZLE ? 4 * c->num_cu : std::min(256, 4 * c->num_cu);
+    // untuned defaults: a grid below 2^22 cells gets one block (16 waves) on the bucket
+    // ring, whose rounds are cheapest (1000^2: 10 ms against 16 ms on 256 heaps); a larger
+    // grid 256 heaps (16 blocks), whose throughput a one-block team cannot match (16384^2:
+    // 195 M expansions); a puzzle 4 heaps per CU, whose open list is wide
+    const int dflt = kind == KIND_PUZZLE ? 4 * c->num_cu
+                                         : cells < (1ll << 22) ? kBlockWaves : std::min(256, 4 * c->num_cu);
     int want = c->cfg.plain_heaps > 0 ? c->cfg.plain_heaps : (tuned > 0 ? tuned : dflt);
     int blocks = std::max(1, (want + kBlockWaves - 1) / kBlockWaves);
     blocks = std::min(blocks, per_cu * c->num_cu);
@@ -378,7 +398,8 @@ extern "C" void ua_grid_free(ua_ctx *c, ua_grid *g) {
 extern "C" int ua_grid_solve(ua_ctx *c, const ua_grid *g, int sx, int sy, int ex, int ey, ua_result *res) {
     if (!c || !res) return fail(c, UA_ERR_INVALID, "null argument");
     c->last_valid = false;
-    int heaps = plain_heaps(c, KIND_GRID);
+    if (!g) return fail(c, UA_ERR_INVALID, "null grid");
+    int heaps = plain_heaps(c, KIND_GRID, (int64_t)g->H * g->W);
     for (int scale = 1; scale <= 64; scale *= 4) {
         QueryDesc q;
         Plan p;
@@ -396,41 +417,76 @@ extern "C" int ua_grid_solve(ua_ctx *c, const ua_grid *g, int sx, int sy, int ex
 
 namespace uart {
 
-// The path of a finished search: start..goal cell ids (grid) or blank moves (puzzle),
-// walked on the device from the goal's parent directions. A found answer fixes the
-// length (orth + diag steps, or the move count), so the walk's buffer is that long.
-int fetch_path(ua_ctx *c, const QueryDesc &q, const QueryResult &r, std::vector<int32_t> &out) {
-    out.clear();
-    if (!r.found || r.status != ST_OK) return fail(c, UA_ERR_INVALID, "query has no path");
-    int64_t want;
-    if (q.kind == KIND_GRID) {
-        int64_t a, b;
-        key_to_pair(r.goal, &a, &b);
-        want = a + b + 1;
-    } else {
-        if (r.rounds == 0) return UA_OK;  // start == goal
-        want = (int64_t)(r.goal >> 32);
+// Paths of finished searches whose state is still on the device: start..goal cell ids
+// (grid) or blank moves (puzzle), walked from each goal's parent directions in one
+// launch, one lane per query. A found answer fixes the length (orth + diag steps, or the
+// move count), so each walk's output is exactly that long. Queries without a path get an
+// empty vector.
+int fetch_paths(ua_ctx *c, int n, const QueryDesc *const *qs, const QueryResult *const *rs,
+                std::vector<int32_t> *out) {
+    std::vector<PathJob> jobs;
+    std::vector<int> which;
+    int64_t total = 0;
+    for (int i = 0; i < n; ++i) {
+        out[i].clear();
+        const QueryDesc &q = *qs[i];
+        const QueryResult &r = *rs[i];
+        if (!r.found || r.status != ST_OK) continue;
+        int64_t want;
+        if (q.kind == KIND_GRID) {
+            int64_t a, b;
+            key_to_pair(r.goal, &a, &b);
+            want = a + b + 1;
+            if (want > (int64_t)q.H * q.W) return fail(c, UA_ERR_HIP, "implausible path length %lld", (long long)want);
+        } else {
+            if (r.rounds == 0) continue;  // start == goal: no moves
+            want = (int64_t)(r.goal >> 32);
+            if (want > 4096) return fail(c, UA_ERR_HIP, "implausible move count %lld", (long long)want);
+        }
+        PathJob j{};
+        j.best = q.best;
+        j.vals = q.vals;
+        j.kind = q.kind;
+        j.n = q.n;
+        j.H = q.H;
+        j.W = q.W;
+        j.sx = q.sx;
+        j.sy = q.sy;
+        j.ex = q.ex;
+        j.ey = q.ey;
+        j.hash_log2 = q.hash_log2;
+        j.goal_slot = r.goal & 0xffffffffull;
+        j.off = total;
+        j.len = want;
+        total += want;
+        jobs.push_back(j);
+        which.push_back(i);
     }
-    if (want < 1 || want > (int64_t)q.H * q.W + 1 + (q.kind == KIND_GRID ? 0 : 1 << 20))
-        return fail(c, UA_ERR_HIP, "implausible path length %lld", (long long)want);
-    const size_t len_off = align256((size_t)want * 4);
-    HIPCHK(c, c->pathbuf.ensure(len_off + 64));
-    int64_t *d_len = reinterpret_cast<int64_t *>(c->pathbuf.as<char>() + len_off);
-    if (q.kind == KIND_GRID)
-        HIPCHK(c, launch_grid_path(q.best, q.H, q.W, q.sx, q.sy, q.ex, q.ey, c->pathbuf.as<int32_t>(), want, d_len,
-                                   c->st));
-    else
-        HIPCHK(c, launch_puzzle_moves(q.n, q.best, q.vals, q.hash_log2, r.goal & 0xffffffffull,
-                                      c->pathbuf.as<int32_t>(), want, d_len, c->st));
-    int64_t n = 0;
-    HIPCHK(c, hipMemcpyAsync(&n, d_len, 8, hipMemcpyDeviceToHost, c->st));
+    if (jobs.empty()) return UA_OK;
+    const size_t jb = align256(jobs.size() * sizeof(PathJob));
+    HIPCHK(c, c->pathbuf.ensure(jb + (size_t)total * 4));
+    PathJob *d_jobs = c->pathbuf.as<PathJob>();
+    int32_t *d_ids = reinterpret_cast<int32_t *>(c->pathbuf.as<char>() + jb);
+    HIPCHK(c, hipMemcpyAsync(d_jobs, jobs.data(), jobs.size() * sizeof(PathJob), hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, launch_paths(d_jobs, (int)jobs.size(), d_ids, c->st));
+    std::vector<int32_t> ids((size_t)total);
+    HIPCHK(c, hipMemcpyAsync(jobs.data(), d_jobs, jobs.size() * sizeof(PathJob), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(ids.data(), d_ids, (size_t)total * 4, hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
-    if (n != want) return fail(c, UA_ERR_HIP, "broken parent chain (%lld of %lld)", (long long)n, (long long)want);
-    std::vector<int32_t> tmp(n);
-    HIPCHK(c, hipMemcpyAsync(tmp.data(), c->pathbuf.p, (size_t)n * 4, hipMemcpyDeviceToHost, c->st));
-    HIPCHK(c, hipStreamSynchronize(c->st));
-    out.assign(tmp.rbegin(), tmp.rend());  // the walk ran goal -> start
+    for (size_t k = 0; k < jobs.size(); ++k) {
+        const PathJob &j = jobs[k];
+        if (j.got != j.len)
+            return fail(c, UA_ERR_HIP, "broken parent chain (%lld of %lld)", (long long)j.got, (long long)j.len);
+        out[which[k]].assign(ids.rbegin() + (total - j.off - j.len), ids.rbegin() + (total - j.off));
+    }
     return UA_OK;
+}
+
+int fetch_path(ua_ctx *c, const QueryDesc &q, const QueryResult &r, std::vector<int32_t> &out) {
+    if (!r.found || r.status != ST_OK) return fail(c, UA_ERR_INVALID, "query has no path");
+    const QueryDesc *qp = &q;
+    const QueryResult *rp = &r;
+    return fetch_paths(c, 1, &qp, &rp, &out);
 }
 
 }  // namespace uart
@@ -663,13 +719,16 @@ static int run_set(ua_ctx *c, int set, std::vector<QueryDesc> &descs, std::vecto
 // launch of their own after the rest); last_kernel_ms is the launches' sum.
 int run_batch(ua_ctx *c, std::vector<QueryDesc> &descs, std::vector<QueryResult> &res, int mode,
               int64_t budget_ticks, const std::vector<QueryResult> *init) {
-    std::vector<int> idx[2];
+    std::vector<int> idx[kSets];
     for (int i = 0; i < (int)descs.size(); ++i) idx[set_of(descs[i])].push_back(i);
-    if (idx[SET_FIVE].empty()) return run_set(c, SET_MAIN, descs, res, mode, budget_ticks, init);
-    if (idx[SET_MAIN].empty()) return run_set(c, SET_FIVE, descs, res, mode, budget_ticks, init);
+    int present = 0, only = SET_MAIN;
+    for (int set = 0; set < kSets; ++set)
+        if (!idx[set].empty()) ++present, only = set;
+    if (present <= 1) return run_set(c, only, descs, res, mode, budget_ticks, init);
     res.assign(descs.size(), QueryResult{});
     double kms = 0;
-    for (int set : {SET_MAIN, SET_FIVE}) {
+    for (int set = 0; set < kSets; ++set) {
+        if (idx[set].empty()) continue;
         std::vector<QueryDesc> d;
         std::vector<QueryResult> r, in;
         for (int i : idx[set]) {
@@ -689,9 +748,21 @@ int run_batch(ua_ctx *c, std::vector<QueryDesc> &descs, std::vector<QueryResult>
 
 extern "C" int ua_batch_solve(ua_ctx *c, const ua_query *qs, int64_t nq, ua_result *out) {
     if (!c || (!qs && nq) || (!out && nq) || nq < 0) return fail(c, UA_ERR_INVALID, "null argument");
+    // The previous batch's state goes; its largest arena stays as a spare for this batch's
+    // first sub-batch that fits in it (allocating gigabytes per call costs tens of ms).
+    DevBuf spare;
+    struct Release {  // the spare is freed on every way out, errors included
+        DevBuf &b;
+        ~Release() { b.release(); }
+    } spare_guard{spare};
+    for (auto &b : c->b_arenas) {
+        if (b.n > spare.n) std::swap(spare, b);
+        b.release();
+    }
+    c->b_arenas.clear();
     size_t free_b = 0, total_b = 0;
     HIPCHK(c, hipMemGetInfo(&free_b, &total_b));
-    size_t budget = c->cfg.mem_budget ? c->cfg.mem_budget : (size_t)(free_b * 0.85);
+    size_t budget = c->cfg.mem_budget ? c->cfg.mem_budget : (size_t)((free_b + spare.n) * 0.85);
 
     // plan every query; reject bad ones up front
     std::vector<QueryDesc> descs(nq);
@@ -732,9 +803,9 @@ extern "C" int ua_batch_solve(ua_ctx *c, const ua_query *qs, int64_t nq, ua_resu
     std::vector<int> scale(nq, 1);
     c->b_desc.assign(nq, QueryDesc{});
     c->b_res.assign(nq, QueryResult{});
-    for (auto &b : c->b_arenas) b.release();
-    c->b_arenas.clear();
     c->b_arena_of.assign(nq, -1);
+    c->b_paths.assign(nq, {});
+    c->b_kept.assign(nq, 0);
     size_t held = 0;
     for (int attempt = 0; attempt < 4 && !todo.empty(); ++attempt) {
         std::vector<int64_t> retry;
@@ -752,15 +823,37 @@ extern "C" int ua_batch_solve(ua_ctx *c, const ua_query *qs, int64_t nq, ua_resu
                 ++pos;
                 continue;
             }
-            if (held + used > budget) {  // release earlier sub-batches' state (paths no longer available)
+            if (held + used > budget) {
+                // release earlier sub-batches' state; their paths are walked first and kept
+                std::vector<const QueryDesc *> rq;
+                std::vector<const QueryResult *> rr;
+                std::vector<int64_t> ri;
+                for (int64_t i = 0; i < nq; ++i)
+                    if (c->b_arena_of[i] >= 0 && c->b_res[i].found && c->b_res[i].status == ST_OK) {
+                        rq.push_back(&c->b_desc[i]);
+                        rr.push_back(&c->b_res[i]);
+                        ri.push_back(i);
+                    }
+                std::vector<std::vector<int32_t>> kept(ri.size());
+                int prc = fetch_paths(c, (int)ri.size(), rq.data(), rr.data(), kept.data());
+                if (prc) return prc;
+                for (size_t k = 0; k < ri.size(); ++k) {
+                    c->b_paths[ri[k]] = std::move(kept[k]);
+                    c->b_kept[ri[k]] = 1;
+                }
                 for (auto &b : c->b_arenas) b.release();
                 for (auto &a : c->b_arena_of) a = -1;
                 held = 0;
             }
             c->b_arenas.emplace_back();
             DevBuf &arena = c->b_arenas.back();
-            HIPCHK(c, arena.ensure(used));
-            held += used;
+            if (spare.p && spare.n >= used) {
+                std::swap(arena, spare);
+            } else {
+                spare.release();
+                HIPCHK(c, arena.ensure(used));
+            }
+            held += arena.n;
             int arena_idx = (int)c->b_arenas.size() - 1;
             char *base = arena.as<char>();
             // layout: [grid best tables | puzzle vals] filled with 0xFF, then [puzzle keys] = 0, then heaps
@@ -830,7 +923,10 @@ extern "C" int ua_batch_solve(ua_ctx *c, const ua_query *qs, int64_t nq, ua_resu
 
 extern "C" int ua_batch_path(ua_ctx *c, int64_t i, int32_t *outp, int64_t cap, int64_t *len) {
     if (!c || !len || i < 0 || i >= (int64_t)c->b_desc.size()) return fail(c, UA_ERR_INVALID, "bad query index");
-    if (c->b_arena_of[i] < 0) return fail(c, UA_ERR_INVALID, "query %lld: search state no longer resident", (long long)i);
+    if (c->b_arena_of[i] < 0) {  // state released by a later sub-batch: the path walked before
+        if (!c->b_kept[i]) return fail(c, UA_ERR_INVALID, "query %lld has no path", (long long)i);
+        return copy_path(c, c->b_paths[i], outp, cap, len);
+    }
     std::vector<int32_t> p;
     int rc = fetch_path(c, c->b_desc[i], c->b_res[i], p);
     if (rc) return rc;

@@ -41,6 +41,7 @@ struct TeamMem {
     uint64_t *seg;     // [2][segcap]
     int segcap;
     uint64_t *stats;   // [5]: pops, expanded, generated, inserted, atomics
+    void *aux;         // LDS of a one-block team (bucket counters of a bucket open list), else null
 };
 
 // Relaxed atomics at a scope chosen at run time. A one-block team is the only user of
@@ -249,6 +250,12 @@ struct GridProb {
     }
     __device__ bool goal_done(uint64_t goal, uint64_t lb) const {
         return goal != SENT && goal <= ((lb >> q.cbits) << q.qshift);
+    }
+    // A popped key whose f is at least the best goal found so far cannot lead to a cheaper
+    // goal (with a consistent h every descendant's f is at least its own): it is dropped
+    // unexpanded. The key's f field rounds down, so the test never drops a cheaper key.
+    __device__ __forceinline__ bool dominated(uint64_t it, uint64_t best_goal) const {
+        return best_goal != SENT && ((it >> q.cbits) << q.qshift) >= best_goal;
     }
     // Octile key from the absolute offsets to the goal (each below 2^31).
     __device__ __forceinline__ static uint64_t oct(uint32_t adx, uint32_t ady) {
@@ -542,6 +549,11 @@ struct PuzzleProb {
     __device__ bool goal_done(uint64_t goal, uint64_t lb) const {
         return goal != SENT && (q.wshift > 0 || (goal >> 32) <= (lb >> 56));
     }
+    // a popped key of f at least the best goal's moves leads to no shorter solution (the
+    // PDB heuristic is consistent): dropped unexpanded
+    __device__ __forceinline__ bool dominated(uint64_t it, uint64_t best_goal) const {
+        return best_goal != SENT && q.wshift == 0 && (it >> 56) >= (best_goal >> 32);
+    }
     template <class SC>
     __device__ __forceinline__ void expand(const SC &sc, uint64_t it, uint64_t (&s)[MAXS], int &cnt,
                                            uint64_t &goalmin, uint32_t &err, Ctr &k) const {
@@ -730,6 +742,11 @@ struct PuzzleProb5 {
     __device__ bool goal_done(uint64_t goal, uint64_t lb) const {
         return goal != SENT && (q.wshift > 0 || (goal >> 32) <= (lb >> 56));
     }
+    // a popped key of f at least the best goal's moves leads to no shorter solution (the
+    // PDB heuristic is consistent): dropped unexpanded
+    __device__ __forceinline__ bool dominated(uint64_t it, uint64_t best_goal) const {
+        return best_goal != SENT && q.wshift == 0 && (it >> 56) >= (best_goal >> 32);
+    }
     template <class SC>
     __device__ __forceinline__ void expand(const SC &sc, uint64_t it, uint64_t (&s)[MAXS], int &cnt,
                                            uint64_t &goalmin, uint32_t &err, Ctr &k) const {
@@ -861,6 +878,9 @@ __device__ __forceinline__ void team_solve(const QueryDesc &q, QueryResult *res,
         uint64_t gk = 0;
         if (it != SENT) {
             ++n_pop;
+            if (P.dominated(it, best_goal)) it = SENT;
+        }
+        if (it != SENT) {
             if constexpr (Prob::kContinue) P.expand(sc, it, s, c, goalmin, err, k, &gk);
             else P.expand(sc, it, s, c, goalmin, err, k);
         }
@@ -1012,6 +1032,387 @@ __device__ __forceinline__ void team_solve(const QueryDesc &q, QueryResult *res,
     }
 }
 
+// ------------------------------------------------------------------ bucket open list (grids)
+// The open list of a grid team as one ring of kBuckets buckets shared by the team, instead
+// of one sorted heap per wave. A key's bucket is its f field (fq = f_key >> qshift) over
+// 2^bshift, i.e. f in steps of 2^(qshift + bshift - 36) (1/16 step as planned); its ring
+// slot is that bucket mod kBuckets. Each ring bucket holds 2^bkt_log2 keys in HBM as a circular array; per ring
+// bucket the team counts the keys appended (tail) and consumed (head). With a consistent
+// heuristic a child's f is at least its parent's and at most 2 sqrt(2) steps more, and a
+// round takes keys only from the lowest bwin buckets, so every key in the open list lies
+// within kBuckets buckets of the lowest one: the ring never aliases (a key outside the
+// window fails the query; it cannot occur). No key is compared with another: a round takes
+// the lowest buckets' keys in arrival order, which replaces the heaps' sorting networks.
+//
+// A round:
+//   (A) every wave reads the ring's counters in window order (bucket cur + 4 lane + i);
+//   barrier (every copy is taken before this round's appends)
+//   (B) plan, identical in every wave: the lowest live bucket blo; the team stops when the
+//       best goal is at most blo's lower f edge (every open key lies at or above it) or
+//       no bucket is live; otherwise it takes up to 64 keys a wave from buckets
+//       blo .. blo + bwin - 1, in bucket and arrival order; the team's first wave writes
+//       the next round's consumed counts (the other copy of head);
+//   (C) lane l of the team's wave w loads the key of rank 64 w + l;
+//   (D) expansion and relaxation as on the heaps (GridProb::expand; continuation when the
+//       window held more keys than the round takes);
+//   (E) each improved successor is appended to its bucket: an atomic add on its tail
+//       gives the position, the key goes to HBM;
+//   barrier (the appends, the round's goal and errors are complete)
+// This is the round of uAstar's kExtractExpand / kDeduplicate / kHeapInsert
+// (pathway/GPU-kernel.cuh:169-544) with its termination test on the device every round.
+//
+// The counters of a one-block team live in LDS (copied to the resume record when a
+// serving launch suspends the query); those of a team of several blocks live at the head
+// of the query's bucket storage and are exchanged at agent scope across the generation
+// barrier.
+struct BucketMeta {
+    uint32_t tail[kBuckets];
+    uint32_t head[2][kBuckets];  // consumed counts: round r reads head[r & 1], writes head[(r + 1) & 1]
+};
+static_assert(sizeof(BucketMeta) <= kBucketMetaWords * 8, "bucket counters fit their reserved words");
+
+template <class Team>
+__device__ __forceinline__ void team_solve_buckets(const QueryDesc &q, QueryResult *res, const Team &team,
+                                                   const GridProb &P, BucketMeta *lds_meta) {
+    const Scope sc = team.sc;
+    const int l = lane_id();
+    const int tw = team.tw, W = team.W;
+    const int tid = (int)threadIdx.x, nthr = (int)blockDim.x;
+    const bool one_block = team.bpg == 1;
+    const bool meta_lds = one_block && lds_meta != nullptr;
+    RoundSlot *const slots = team.m.slot;
+    uint64_t *const stats = team.m.stats;
+    if (P.trivial()) {  // start == goal: cost 0, no search
+        if (team.leader()) {
+            res->status = ST_OK;
+            res->found = 1;
+            res->goal = 0;
+            res->rounds = 0;
+            res->pops = res->expanded = res->generated = res->inserted = res->atomics = 0;
+            res->flags = 0;
+        }
+        return;
+    }
+    constexpr uint32_t RM = kBuckets - 1;
+    const int bsh = q.cbits + q.bshift;  // key >> bsh = the key's bucket
+    const int blog = q.bkt_log2;
+    const uint32_t bmask = (1u << blog) - 1, bcap = 1u << blog;
+    BucketMeta *const bm = meta_lds ? lds_meta : reinterpret_cast<BucketMeta *>(q.heap);
+    UA_GLOBAL uint64_t *const store = gptr(q.heap) + kBucketMetaWords;
+    const bool resumed = q.resume != nullptr && q.resume[0] != 0;
+    const int r0 = resumed ? (int)q.resume[1] : 0;
+    // resume record after its header: the lowest bucket, then (counters in LDS) tail[] and
+    // the next round's consumed counts as 32-bit words
+    uint32_t *const rec32 = q.resume ? reinterpret_cast<uint32_t *>(q.resume + RESUME_HEADER + 1) : nullptr;
+    uint64_t cur;  // the lowest bucket that can hold a key (wave-uniform)
+    if (resumed) {
+        cur = q.resume[RESUME_HEADER];
+        if (meta_lds)
+            for (int i = tid; i < kBuckets; i += nthr) {
+                bm->tail[i] = rec32[i];
+                bm->head[r0 & 1][i] = rec32[kBuckets + i];
+            }
+    } else {
+        cur = P.item(octile_key(q.sx, q.sy, q.ex, q.ey), q.sx, q.sy) >> bsh;
+        if (team.gblock == 0)
+            for (int i = tid; i < kBuckets; i += nthr) {
+                sc.store(&bm->tail[i], 0u);
+                sc.store(&bm->head[0][i], 0u);
+            }
+    }
+    if (team.leader()) {
+        for (int i = 0; i < 2; ++i) {
+            sc.store(&slots[i].lb, SENT);
+            sc.store(&slots[i].goal, SENT);
+            sc.store(&slots[i].cnt, 0u);
+            sc.store(&slots[i].err, 0u);
+        }
+        for (int i = 0; i < 4; ++i) sc.store(&stats[i], resumed ? q.resume[3 + i] : (uint64_t)0);
+        sc.store(&stats[4], resumed ? q.resume[8] : (uint64_t)0);
+    }
+    if (!team.sync([] {})) return;
+    if (!resumed && team.leader()) {  // the start's record and key
+        const uint64_t key0 = P.seed(sc);
+        const uint32_t ring = (uint32_t)cur & RM;
+        store[(uint64_t)ring << blog] = key0;
+        sc.store(&bm->tail[ring], 1u);
+    }
+    if (!team.sync([] {})) return;
+
+    uint32_t n_pop = 0;
+    Ctr k;
+    uint64_t best_goal = resumed ? q.resume[2] : SENT;
+    uint32_t pruned = resumed ? (uint32_t)q.resume[7] & FLAG_PRUNED : 0u;
+    int32_t status = ST_OK;
+    int found = 0;
+    const uint32_t pmax = (uint32_t)W * 64;  // keys a round takes at most
+    const int win = q.bwin < 1 ? 1 : q.bwin;
+    TeamVars *const tvp = team.tv;
+    int r = r0;
+    for (;; ++r) {
+        RoundSlot *const cs = &slots[r & 1];
+        uint32_t *const hcur = bm->head[r & 1];
+        uint32_t *const hnext = bm->head[(r + 1) & 1];
+        // (A) the ring's counters in window order: element i of lane l is bucket cur + 4 l + i
+        uint32_t hd[4], lv[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const uint32_t ring = ((uint32_t)cur + 4u * (uint32_t)l + (uint32_t)i) & RM;
+            const uint32_t t = sc.load(&bm->tail[ring]);
+            hd[i] = sc.load(&hcur[ring]);
+            lv[i] = t - hd[i];
+        }
+        if (one_block) {
+            __syncthreads();
+        } else if (!team.sync([] {})) {
+            status = ST_ABORT;
+            break;
+        }
+        // (B) plan
+        int min_d = kBuckets;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const uint64_t m = __ballot(lv[i] != 0);
+            if (m) {
+                const int d = 4 * __builtin_ctzll(m) + i;
+                min_d = d < min_d ? d : min_d;
+            }
+        }
+        if (min_d == kBuckets) {  // no live bucket: the open list is exhausted
+            found = best_goal != SENT;
+            status = found ? ST_OK : ST_NO_PATH;
+            break;
+        }
+        const uint64_t blo = cur + (uint64_t)min_d;
+        if (best_goal != SENT && best_goal <= ((blo << q.bshift) << q.qshift)) {  // goal <= every open f
+            found = 1;
+            break;
+        }
+        if (r + 1 >= q.max_rounds) {
+            status = ST_ROUNDS;
+            break;
+        }
+        if (team.leader()) {  // the next round's slot, and this round's time budget
+            RoundSlot *const ns = &slots[(r + 1) & 1];
+            sc.store(&ns->lb, SENT);
+            sc.store(&ns->goal, SENT);
+            sc.store(&ns->cnt, 0u);
+            sc.store(&ns->err, 0u);
+            const uint64_t dl = __hip_atomic_load(&tvp->deadline, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (dl && (uint64_t)now_ticks() > dl) sc.fold_or(&cs->err, (uint32_t)FLAG_TIMEUP);
+        }
+        // the window's keys, clamped to pmax in window order
+        uint32_t tk[4], lsum = 0;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int d = 4 * l + i;
+            tk[i] = (d >= min_d && d < min_d + win) ? lv[i] : 0u;
+            lsum += tk[i];
+        }
+        int tot;
+        const uint32_t ex = (uint32_t)wave_excl_scan((int)lsum, &tot);
+        const uint32_t total = (uint32_t)tot < pmax ? (uint32_t)tot : pmax;
+        uint32_t run = ex;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const uint32_t lo = run < pmax ? run : pmax;
+            run += tk[i];
+            const uint32_t hi = run < pmax ? run : pmax;
+            tk[i] = hi - lo;
+        }
+        const uint32_t exc = ex < pmax ? ex : pmax;  // this lane's first rank
+        if (tw == 0) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                sc.store(&hnext[((uint32_t)cur + 4u * (uint32_t)l + (uint32_t)i) & RM], hd[i] + tk[i]);
+        }
+        // (C) the key of rank j = 64 w + l: the lane L that holds it (the largest L with
+        // exc_L <= j), then its element
+        const uint32_t j = (uint32_t)tw * 64u + (uint32_t)l;
+        int L = 0;
+#pragma unroll
+        for (int st = 32; st > 0; st >>= 1) {
+            const int c = L + st;  // at most 63
+            if ((uint32_t)__shfl((int)exc, c) <= j) L = c;
+        }
+        uint32_t off = j - (uint32_t)__shfl((int)exc, L);
+        const uint32_t t0 = (uint32_t)__shfl((int)tk[0], L), t1 = (uint32_t)__shfl((int)tk[1], L),
+                       t2 = (uint32_t)__shfl((int)tk[2], L);
+        const uint32_t h0 = (uint32_t)__shfl((int)hd[0], L), h1 = (uint32_t)__shfl((int)hd[1], L),
+                       h2 = (uint32_t)__shfl((int)hd[2], L), h3 = (uint32_t)__shfl((int)hd[3], L);
+        int e = 0;
+        uint32_t hsel = h0;
+        if (off >= t0) {
+            off -= t0;
+            e = 1;
+            hsel = h1;
+            if (off >= t1) {
+                off -= t1;
+                e = 2;
+                hsel = h2;
+                if (off >= t2) {
+                    off -= t2;
+                    e = 3;
+                    hsel = h3;
+                }
+            }
+        }
+        uint64_t it = SENT;
+        if (j < total) {
+            const uint32_t ring = ((uint32_t)cur + 4u * (uint32_t)L + (uint32_t)e) & RM;
+            it = store[((uint64_t)ring << blog) | ((hsel + off) & bmask)];
+        }
+        const uint32_t cur0 = (uint32_t)cur;  // the snapshot's first ring bucket
+        cur = blo;
+        // (D) expansion
+        uint64_t s[GridProb::MAXS];
+#pragma unroll
+        for (int i = 0; i < GridProb::MAXS; ++i) s[i] = SENT;
+        int c = 0;
+        uint64_t goalmin = SENT;
+        uint32_t err = 0;
+        uint64_t gk = 0;
+        if (it != SENT) {
+            ++n_pop;
+            if (P.dominated(it, best_goal)) it = SENT;
+        }
+        if (it != SENT) P.expand(sc, it, s, c, goalmin, err, k, &gk);
+        // continuation only in a saturated round (the window held more keys than the round
+        // takes), so that the chains run ahead of a best-first frontier
+        uint64_t keep = SENT;
+        int dc = 0;
+        const uint64_t child = it != SENT && (uint32_t)tot > pmax ? P.pick_child(it, s, c, &dc) : SENT;
+        if (__ballot(child != SENT)) {
+            if (child != SENT) {  // the child's sibling, if any, waits in `keep`
+#pragma unroll
+                for (int i = 0; i < GridProb::MAXS; ++i) {
+                    if (i != dc && s[i] < keep) keep = s[i];
+                    s[i] = SENT;
+                }
+                c = keep != SENT;
+                P.expand_child(sc, child, gk + (dc < 4 ? KEY_ONE : KEY_SQRT2), s, c, goalmin, err, k);
+            }
+        }
+        // (E) appends (s[] and the sibling kept by continuation): every key's tail add is
+        // issued before any of the positions is used
+        constexpr int NK = GridProb::MAXS + 1;
+        uint32_t pos[NK];
+#pragma unroll
+        for (int i = 0; i < NK; ++i) {
+            uint64_t &key = i < GridProb::MAXS ? s[i] : keep;
+            pos[i] = 0;
+            if (key == SENT) continue;
+            const uint64_t b = key >> bsh;
+            if (b - cur >= (uint64_t)kBuckets) {  // outside the ring's window: impossible with a consistent h
+                err |= FLAG_ABORT;
+                key = SENT;
+                continue;
+            }
+            pos[i] = sc.fadd(&bm->tail[(uint32_t)b & RM], 1u);
+        }
+#pragma unroll
+        for (int i = 0; i < NK; ++i) {
+            const uint64_t key = i < GridProb::MAXS ? s[i] : keep;
+            const uint32_t ring = (uint32_t)(key >> bsh) & RM;
+            // the ring bucket's consumed count at this round's start: from LDS, or from the
+            // lane of the snapshot that holds it (every lane runs the shuffles)
+            uint32_t h;
+            if (meta_lds) {
+                h = key != SENT ? bm->head[r & 1][ring] : 0u;
+            } else {
+                const uint32_t d = (ring - cur0) & RM;
+                const uint32_t a0 = (uint32_t)__shfl((int)hd[0], (int)(d >> 2)),
+                               a1 = (uint32_t)__shfl((int)hd[1], (int)(d >> 2)),
+                               a2 = (uint32_t)__shfl((int)hd[2], (int)(d >> 2)),
+                               a3 = (uint32_t)__shfl((int)hd[3], (int)(d >> 2));
+                h = (d & 3) == 0 ? a0 : (d & 3) == 1 ? a1 : (d & 3) == 2 ? a2 : a3;
+            }
+            if (key == SENT) continue;
+            if (pos[i] - h >= bcap) {  // full: at least this many of its keys are unconsumed
+                err |= FLAG_CAPACITY;
+                continue;
+            }
+            store[((uint64_t)ring << blog) | (pos[i] & bmask)] = key;
+        }
+        if (__ballot(goalmin != SENT)) {
+            goalmin = wave_min64(goalmin);
+            if (l == 0) sc.fmin(&cs->goal, goalmin);
+        }
+        if (__ballot(err != 0)) {
+            for (int m = 32; m > 0; m >>= 1) err |= __shfl_xor(err, m);
+            if (l == 0) sc.fold_or(&cs->err, err);
+        }
+        uint64_t g;
+        uint32_t rerr;
+        if (one_block) {  // the slot is in LDS: every wave reads the same values after the barrier
+            __syncthreads();
+            g = cs->goal;
+            rerr = cs->err;
+        } else {  // one lane reads the slot for the block, so every wave decides on the same values
+            if (!team.sync([] {})) {
+                status = ST_ABORT;
+                break;
+            }
+            RoundSlot *const view = team.m.view;
+            if (tid == 0) {
+                view->goal = sc.load(&cs->goal);
+                view->err = sc.load(&cs->err);
+            }
+            __syncthreads();
+            g = view->goal;
+            rerr = view->err;
+        }
+        best_goal = best_goal < g ? best_goal : g;
+        pruned |= rerr & FLAG_PRUNED;
+        if (rerr & ~(uint32_t)(FLAG_TIMEUP | FLAG_PRUNED)) {
+            status = (rerr & FLAG_ABORT) ? ST_ABORT : (rerr & FLAG_CAPACITY) ? ST_CAPACITY : ST_RANGE;
+            break;
+        }
+        if (q.resume != nullptr && ((rerr & FLAG_TIMEUP) || (q.round_budget > 0 && r + 1 - r0 >= q.round_budget))) {
+            status = ST_SUSPENDED;  // every wave reaches this with the same r and slot values
+            break;
+        }
+    }
+    // ---- statistics, resume record and result
+    const int64_t t_pop = wave_sum64(n_pop), t_exp = wave_sum64(k.exp), t_gen = wave_sum64(k.gen),
+                  t_ins = wave_sum64(k.ins), t_atom = wave_sum64(k.atom);
+    if (l == 0) {
+        sc.fadd(&stats[0], (uint64_t)t_pop);
+        sc.fadd(&stats[1], (uint64_t)t_exp);
+        sc.fadd(&stats[2], (uint64_t)t_gen);
+        sc.fadd(&stats[3], (uint64_t)t_ins);
+        sc.fadd(&stats[4], (uint64_t)t_atom);
+    }
+    if (status == ST_SUSPENDED && meta_lds) {  // the ring after round r, with the next round's consumed counts
+        for (int i = tid; i < kBuckets; i += nthr) {
+            rec32[i] = bm->tail[i];
+            rec32[kBuckets + i] = bm->head[(r + 1) & 1][i];
+        }
+    }
+    team.sync([] {});
+    if (team.leader()) {
+        if (status == ST_SUSPENDED) {
+            q.resume[1] = (uint64_t)(r + 1);
+            q.resume[2] = best_goal;
+            for (int i = 0; i < 4; ++i) q.resume[3 + i] = sc.load(&stats[i]);
+            q.resume[7] = pruned;
+            q.resume[8] = sc.load(&stats[4]);
+            q.resume[RESUME_HEADER] = cur;
+            q.resume[0] = 1;
+        }
+        res->status = status;
+        res->found = found;
+        res->goal = best_goal;
+        res->rounds = r + 1;
+        res->pops = (int64_t)sc.load(&stats[0]);
+        res->expanded = (int64_t)sc.load(&stats[1]);
+        res->generated = (int64_t)sc.load(&stats[2]);
+        res->inserted = (int64_t)sc.load(&stats[3]) + 1;
+        res->atomics = (int64_t)sc.load(&stats[4]);
+        res->flags = pruned;
+    }
+}
+
 // A query of the other problem set (set_of) fails with ST_INVALID.
 template <int SET, class Team>
 __device__ __forceinline__ void dispatch(const QueryDesc &q, QueryResult *res, const Team &team) {
@@ -1025,9 +1426,12 @@ __device__ __forceinline__ void dispatch(const QueryDesc &q, QueryResult *res, c
     if constexpr (SET == SET_FIVE) {
         PuzzleProb5 P(q);
         team_solve(q, res, team, P);
-    } else if (q.kind == KIND_GRID) {
+    } else if constexpr (SET == SET_GHEAP) {
         GridProb P(q);
         team_solve(q, res, team, P);
+    } else if (q.kind == KIND_GRID) {
+        GridProb P(q);
+        team_solve_buckets(q, res, team, P, static_cast<BucketMeta *>(team.m.aux));
     } else {
         PuzzleProb P(q, q.pdb);
         team_solve(q, res, team, P);
@@ -1105,6 +1509,10 @@ __device__ __forceinline__ void group_block(const QueryDesc *qs, QueryResult *rs
     m.stats = &tv->n_pop;
     m.seg = in_lds ? lds_seg : segs + seg_off;
     m.segcap = segcap;
+    // a one-block bucket team's counters take the LDS insert buffer's place (that path does
+    // not use the buffer)
+    static_assert(sizeof(BucketMeta) <= sizeof(lds_seg), "bucket counters fit the LDS buffer");
+    m.aux = in_lds ? static_cast<void *>(lds_seg) : nullptr;
     GroupTeam team(m, tv, bpg, rel % bpg);
     group_loop<SET>(qs, rs, team, tv, qhead + cls, shape.nq[cls], shape.qoff[cls], shape.budget_ticks);
 }

@@ -144,7 +144,7 @@ int run_step(ua_ctx *c, int mode, int set, int blocks, const LaunchShape &shape,
              const QueryResult *init, int nq, QueryResult *out) {
     using Clock = std::chrono::steady_clock;
     const auto t0 = Clock::now();
-    if (mode < 0 || mode > 2 || (set != SET_MAIN && set != SET_FIVE))
+    if (mode < 0 || mode > 2 || set < 0 || set >= kSets)
         return fail(c, UA_ERR_INVALID, "bad launch mode %d / set %d", mode, set);
     int per_cu = 0;
     HIPCHK(c, teams_max_blocks(&per_cu));
