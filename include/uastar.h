@@ -112,6 +112,18 @@ int ua_grid_solve(ua_ctx *ctx, const ua_grid *g, int sx, int sy, int ex, int ey,
  * GPUPathwaySolver::getSolution (pathway/GPU-solver.cu:292-316). */
 int ua_grid_path(ua_ctx *ctx, int32_t *ids, int64_t cap, int64_t *len);
 
+/* ua_grid_solve and ua_grid_path in one call that allocates nothing: no counterpart in the
+ * reference. For callers that cannot allocate device memory (code under hipGraph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). Every device buffer of the call is
+ * a region of one static array of 46,158,080 bytes (44 MiB) per GPU, held until the process
+ * ends; the grid may have
+ * at most 2^20 cells (UA_ERR_INVALID otherwise, before any device work). The solve runs one
+ * block on the bucket ring; a ring bucket that fills up returns UA_ERR_CAPACITY (no retry).
+ * With len != NULL the path (start..goal cell ids) goes to path[0..*len); path == NULL asks
+ * for the length only. Calls that overlap on one GPU share the array and race. */
+int ua_grid_solve_noalloc(ua_ctx *ctx, const ua_grid *g, int sx, int sy, int ex, int ey, ua_result *res,
+                          int32_t *path, int64_t cap, int64_t *len);
+
 /* ---- sliding-tile puzzles, n = 3, 4 or 5. */
 /* Builds the reference's additive disjoint PDB on the device (puzzle/database.cpp:37-84;
  * partitions of puzzle/CPU-solver.hpp:44-104: one 8-tile table for n = 3, 8 + 7 tiles for

@@ -98,6 +98,8 @@ def lib():
             "ua_grid_free": (None, [vp, vp]),
             "ua_grid_solve": (C.c_int, [vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(Result)]),
             "ua_grid_path": (C.c_int, [vp, vp, i64, C.POINTER(i64)]),
+            "ua_grid_solve_noalloc": (C.c_int, [vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(Result), vp,
+                                                i64, C.POINTER(i64)]),
             "ua_puzzle_build_pdb": (C.c_int, [vp, C.c_int]),
             "ua_puzzle_download_pdb": (C.c_int, [vp, C.c_int, vp, u64]),
             "ua_puzzle_load_pdb": (C.c_int, [vp, C.c_int, vp, u64]),
@@ -252,6 +254,18 @@ class Context:
         r = Result()
         self._check(lib().ua_grid_solve(self.h, grid.handle, sx, sy, ex, ey, C.byref(r)), (UA_OK, UA_NO_PATH))
         return r.as_dict()
+
+    def grid_solve_noalloc(self, grid: Grid, sx, sy, ex, ey, want_path=True):
+        """ua_grid_solve_noalloc: the plain solve and its path with no device allocation
+        (grids of at most 2^20 cells). Returns (result dict, path or None)."""
+        r = Result()
+        n = C.c_int64()
+        cap = grid.H * grid.W if want_path else 0
+        ids = np.empty(max(1, cap), np.int32)
+        rc = lib().ua_grid_solve_noalloc(self.h, grid.handle, sx, sy, ex, ey, C.byref(r),
+                                         ids.ctypes.data if want_path else None, cap, C.byref(n))
+        self._check(rc, (UA_OK, UA_NO_PATH))
+        return r.as_dict(), (ids[: n.value].copy() if want_path and r.found else None)
 
     def grid_path(self) -> np.ndarray:
         n = C.c_int64()

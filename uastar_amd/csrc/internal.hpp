@@ -88,6 +88,9 @@ struct StepGraph {
 struct ua_ctx {
     int dev = 0;
     hipStream_t st = nullptr;
+    // device address of this GPU's copy of the allocation-free entry point's scratch array
+    // (runtime.cpp), from hipGetSymbolAddress at ua_create; null if that call failed
+    char *noalloc_scratch = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;  // around each search launch on st
     double last_kernel_ms = 0;
     ua_config cfg{};

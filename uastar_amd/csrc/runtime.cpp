@@ -24,6 +24,22 @@
 using namespace ua;
 using namespace uart;
 
+// Scratch of ua_grid_solve_noalloc: one array per GPU, held until the process ends. Host
+// code uses it only through the device address hipGetSymbolAddress gives at ua_create
+// (ua_ctx::noalloc_scratch); the name itself is a host-side stand-in. Layout, for a grid of
+// at most kNoAllocCells cells: the launch's I/O block, the path walk's job and ids, the best
+// table (8 B a cell), then the bucket ring (its counters and 256 x 2^kNoAllocBktLog2 keys).
+namespace {
+constexpr int64_t kNoAllocCells = 1ll << 20;
+constexpr int kNoAllocBktLog2 = 14;
+constexpr size_t kNoAllocIo = 16384;                                    // I/O block and path job
+constexpr size_t kNoAllocPath = (size_t)kNoAllocCells * 4 + 256;        // path ids
+constexpr size_t kNoAllocBest = (size_t)kNoAllocCells * 8;              // best table
+constexpr size_t kNoAllocRing = ((size_t)kBucketMetaWords + ((size_t)kBuckets << kNoAllocBktLog2)) * 8;
+constexpr size_t kNoAllocBytes = kNoAllocIo + kNoAllocPath + kNoAllocBest + kNoAllocRing;
+}  // namespace
+static __device__ __align__(256) uint64_t scratch[(kNoAllocBytes + 7) / 8];
+
 namespace uart {
 
 int fail(ua_ctx *c, int code, const char *fmt, ...) {
@@ -300,6 +316,11 @@ extern "C" int ua_create(int ordinal, const ua_config *cfg, ua_ctx **out) {
         c->total_mem = prop.totalGlobalMem;
     }
     if (c->num_cu <= 0) c->num_cu = 1;
+    // this device's copy of the allocation-free solve's scratch array (each GPU holds its
+    // own, at its own address); without it that entry point reports an error
+    void *sp = nullptr;
+    if (hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)) == hipSuccess) c->noalloc_scratch = static_cast<char *>(sp);
+    else (void)hipGetLastError();
     *out = c;
     return UA_OK;
 }
@@ -508,6 +529,98 @@ extern "C" int ua_grid_path(ua_ctx *c, int32_t *ids, int64_t cap, int64_t *len) 
     int rc = fetch_path(c, c->last_q, c->last_r, p);
     if (rc) return rc;
     return copy_path(c, p, ids, cap, len);
+}
+
+// The plain grid solve and its path with no allocation: every device buffer of the call is
+// a region of the scratch array, and the I/O block is staged from the host stack. One
+// block (16 waves) on the bucket ring, as ua_grid_solve runs grids below 2^22 cells untuned.
+// Calls that overlap on one GPU share the array and race.
+extern "C" int ua_grid_solve_noalloc(ua_ctx *c, const ua_grid *g, int sx, int sy, int ex, int ey, ua_result *res,
+                                     int32_t *path, int64_t cap, int64_t *len) {
+    if (!c || !g || !res) return fail(c, UA_ERR_INVALID, "null argument");
+    c->last_valid = false;
+    std::memset(res, 0, sizeof *res);
+    if (len) *len = 0;
+    const int64_t cells = (int64_t)g->H * g->W;
+    if (cells > kNoAllocCells)
+        return fail(c, UA_ERR_INVALID, "ua_grid_solve_noalloc holds grids of at most %lld cells (this one has %lld)",
+                    (long long)kNoAllocCells, (long long)cells);
+    if (!c->noalloc_scratch) return fail(c, UA_ERR_HIP, "the scratch array's device address is unknown");
+    QueryDesc q;
+    Plan p;
+    int rc = grid_fill_desc(c, g, sx, sy, ex, ey, kBlockWaves, 1, q, p);
+    if (rc) return rc;
+    char *const base = c->noalloc_scratch;
+    char *const io = base;
+    int32_t *const d_path = reinterpret_cast<int32_t *>(base + kNoAllocIo);
+    q.best = reinterpret_cast<uint64_t *>(base + kNoAllocIo + kNoAllocPath);
+    q.heap = reinterpret_cast<uint64_t *>(base + kNoAllocIo + kNoAllocPath + kNoAllocBest);
+    q.vals = nullptr;
+    q.open_kind = OPEN_BUCKETS;  // whatever the context's flags: the scratch holds a ring
+    q.bshift = std::max(0, 32 - q.qshift);
+    q.bwin = std::max(1, std::min(c->tune.grid_bucket_window, kBuckets - 48));
+    q.bkt_log2 = kNoAllocBktLog2;
+    q.heap_cap = 0;
+    HIPCHK(c, hipMemsetAsync(q.best, 0xFF, (size_t)cells * 8, c->st));
+    HIPCHK(c, launch_grid_mark_blocked(q.best, q.bitmap, q.H, q.W, q.wpr, c->st));
+    // the I/O block of one one-block group: header, TeamVars (zero), descriptor, result
+    const size_t tv_off = align256(sizeof(StepHeader));
+    const size_t qd_off = tv_off + align256(sizeof(TeamVars));
+    const size_t qr_off = qd_off + align256(sizeof(QueryDesc));
+    const size_t end = qr_off + sizeof(QueryResult);
+    static_assert(3 * 256 + sizeof(StepHeader) + sizeof(TeamVars) + sizeof(QueryDesc) + sizeof(QueryResult) +
+                          sizeof(PathJob) + 256 <= kNoAllocIo,
+                  "the I/O block and the path job fit their region");
+    alignas(256) char blk[kNoAllocIo];
+    std::memset(blk, 0, end);
+    StepHeader h{};
+    h.shape.groups[0] = 1;
+    for (int k = 0; k < kClasses; ++k) h.shape.bpg[k] = 1;
+    h.shape.nq[0] = 1;
+    h.shape.segcap[0] = kBlockWaves * 64 * 8;  // the LDS buffer's size: the group keeps it in LDS
+    h.tv_off = (uint32_t)tv_off;
+    h.qd_off = (uint32_t)qd_off;
+    h.qr_off = (uint32_t)qr_off;
+    h.nq = 1;
+    std::memcpy(blk, &h, sizeof h);
+    std::memcpy(blk + qd_off, &q, sizeof q);
+    HIPCHK(c, hipMemcpyAsync(io, blk, end, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipEventRecord(c->ev0, c->st));
+    HIPCHK(c, launch_teams(io, 1, reinterpret_cast<uint64_t *>(base), 0, SET_MAIN, c->st));
+    HIPCHK(c, hipEventRecord(c->ev1, c->st));
+    QueryResult r{};
+    HIPCHK(c, hipMemcpyAsync(&r, io + qr_off, sizeof r, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    float kms = 0;
+    HIPCHK(c, hipEventElapsedTime(&kms, c->ev0, c->ev1));
+    fill_result(q, r, res);
+    res->kernel_ms = kms;
+    if (res->status < 0) return fail(c, res->status, "grid solve failed with status %d", res->status);
+    if (res->status != UA_OK || !len) return res->status;
+    // the path: one walk into the scratch, then reversed into the caller's buffer
+    const int64_t want = res->orth + res->diag + 1;
+    if (want > cap && path) return fail(c, UA_ERR_INVALID, "path needs %lld entries", (long long)want);
+    *len = want;
+    if (!path) return res->status;
+    PathJob j{};
+    j.best = q.best;
+    j.kind = KIND_GRID;
+    j.H = q.H;
+    j.W = q.W;
+    j.sx = q.sx;
+    j.sy = q.sy;
+    j.ex = q.ex;
+    j.ey = q.ey;
+    j.len = want;
+    PathJob *const d_job = reinterpret_cast<PathJob *>(io + align256(end));
+    HIPCHK(c, hipMemcpyAsync(d_job, &j, sizeof j, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, launch_paths(d_job, 1, d_path, c->st));
+    HIPCHK(c, hipMemcpyAsync(&j, d_job, sizeof j, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(path, d_path, (size_t)want * 4, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    if (j.got != want) return fail(c, UA_ERR_HIP, "broken parent chain (%lld of %lld)", (long long)j.got, (long long)want);
+    std::reverse(path, path + want);  // the walk ran goal -> start
+    return res->status;
 }
 
 extern "C" int ua_puzzle_build_pdb(ua_ctx *c, int n) {
