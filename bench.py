@@ -345,7 +345,7 @@ def cpu_baseline_mixed(ctx, specs, budget_s=25.0, threads=None, seed=11, puzzle_
 
 def run_serve_mixed(args):
     import uastar_amd as U
-    ctx = U.Context(0, graphs=args.graph, grid_heaps=args.grid_heaps)
+    ctx = U.Context(0, graphs=args.graph, grid_heaps=args.grid_heaps, xcd_shared=args.xcd_shared)
     n_warm, n_timed = args.warmup * QUERIES_PER_STEP, args.steps * QUERIES_PER_STEP
     specs = mixed_stream(n_warm + n_timed)
     materialize(ctx, specs)
@@ -414,6 +414,7 @@ def run_serve_mixed(args):
                                "load x plain capacity",
                    "queries": len(specs), "queries_per_step": QUERIES_PER_STEP,
                    "mode": "serve (continuous batching; team width per class from the autotuner)",
+                   "xcds": "one class per team scope" if not args.xcd_shared else "shared",
                    "grid_open_list": "per-wave heaps" if args.grid_heaps else "bucket ring",
                    "load": args.load,
                    "arrival_rate_qps": rate},
@@ -553,6 +554,8 @@ def main():
                     help="heaps (waves) of the plain whole-GPU solve of the plain workloads (0 = default)")
     ap.add_argument("--grid-heaps", action="store_true",
                     help="grids keep one sorted heap per wave instead of the team's bucket ring (comparison)")
+    ap.add_argument("--xcd-shared", action="store_true",
+                    help="serving: every team class on every XCD (no XCDs apart for one-block grid teams)")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (rocprofv3 counter calibration)")
     args = ap.parse_args()
@@ -603,8 +606,8 @@ def main():
                                 "start (0,0), goal (16383,16383), one query per whole-GPU launch") if big else
                                ("plain GA*: 1000x1000 grids, Bernoulli(0.30) obstacles from seeds 1.., "
                                 "start (0,0), goal (999,999), one query per whole-GPU launch"),
-                   "mode": "plain", "heaps": ctx.get_tuning()["plain_grid_heaps"],
-                   "open_list": "per-wave heaps" if args.grid_heaps else "bucket ring"},
+                   "mode": "plain", "heaps": sorted({r["heaps"] for r in res}),
+                   "open_list": "per-wave heaps" if args.grid_heaps or res[0]["heaps"] > 16 else "bucket ring"},
         "rounds_p50": pct([r["rounds"] for r in res], 50),
         "answers": [(r["orth"], r["diag"]) for r in res[:4]],
         "p50_ms": pct(lat, 50),

@@ -62,6 +62,10 @@ typedef struct {
 /* ua_config.flags: grids on one-block teams keep one sorted heap per wave instead of the
  * team's bucket ring (for comparisons; multi-block grid teams always use heaps) */
 #define UA_FLAG_GRID_HEAPS 4
+/* ua_config.flags: batch and serving launches spread every class over every XCD instead of
+ * giving one-block grid teams XCDs (and so L2 caches) apart from the multi-block puzzle
+ * teams (for comparisons) */
+#define UA_FLAG_XCD_SHARED 8
 
 typedef struct {
     int32_t status;
@@ -78,6 +82,9 @@ typedef struct {
     int32_t weight;      /* weight on h of the search that found the answer (1 = plain A*) */
     int64_t atomics;     /* device-memory atomics the search issued on its table or hash (relaxing
                             atomicMin, slot-claiming CAS) */
+    int32_t heaps;       /* waves of the team that searched the query (= its heaps; a one-block grid
+                            team shares one bucket ring) */
+    int32_t reserved;
 } ua_result;
 
 /* Device context: owns every device allocation. Replaces the device set-up of
@@ -219,7 +226,8 @@ typedef struct {
 int ua_autotune(ua_ctx *ctx, const ua_query *sample, int64_t n, ua_tuning *out);
 /* the settings in effect (tuned, configured or default) */
 int ua_get_tuning(ua_ctx *ctx, ua_tuning *out);
-/* applies stored settings without measuring (e.g. from an earlier ua_autotune) */
+/* applies stored settings without measuring (e.g. from an earlier ua_autotune); a plain
+ * heap count of 0 restores the untuned default of its kind */
 int ua_set_tuning(ua_ctx *ctx, const ua_tuning *t);
 
 /* ---- diagnostics */

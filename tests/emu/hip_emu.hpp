@@ -120,6 +120,7 @@ inline uint64_t __builtin_amdgcn_s_memrealtime() {
     return (uint64_t)std::chrono::steady_clock::now().time_since_epoch().count() / 10;
 }
 inline void __builtin_amdgcn_s_sleep(int) { std::this_thread::yield(); }
+inline void __builtin_amdgcn_s_waitcnt(int) { std::atomic_thread_fence(std::memory_order_seq_cst); }
 #define __builtin_amdgcn_fence(order, scope) std::atomic_thread_fence(std::memory_order_seq_cst)
 
 template <class T, class U>

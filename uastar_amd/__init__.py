@@ -44,6 +44,7 @@ class Config(C.Structure):
 FLAG_GRAPH = 1
 FLAG_SERVE_PATHS = 2
 FLAG_GRID_HEAPS = 4
+FLAG_XCD_SHARED = 8
 
 
 class Result(C.Structure):
@@ -51,7 +52,8 @@ class Result(C.Structure):
                 ("steps", C.c_int64), ("cost", C.c_double), ("rounds", C.c_int64), ("pops", C.c_int64),
                 ("expanded", C.c_int64), ("generated", C.c_int64), ("inserted", C.c_int64),
                 ("device_ms", C.c_double), ("kernel_ms", C.c_double), ("latency_ms", C.c_double),
-                ("optimal", C.c_int32), ("weight", C.c_int32), ("atomics", C.c_int64)]
+                ("optimal", C.c_int32), ("weight", C.c_int32), ("atomics", C.c_int64), ("heaps", C.c_int32),
+                ("reserved", C.c_int32)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -204,9 +206,9 @@ class Context:
     """One device context (ua_ctx)."""
 
     def __init__(self, ordinal=0, team_waves=0, plain_heaps=0, max_rounds=0, hash_log2=0, mem_budget=0,
-                 puzzle_team_waves=0, graphs=False, serve_paths=False, grid_heaps=False):
+                 puzzle_team_waves=0, graphs=False, serve_paths=False, grid_heaps=False, xcd_shared=False):
         flags = ((FLAG_GRAPH if graphs else 0) | (FLAG_SERVE_PATHS if serve_paths else 0) |
-                 (FLAG_GRID_HEAPS if grid_heaps else 0))
+                 (FLAG_GRID_HEAPS if grid_heaps else 0) | (FLAG_XCD_SHARED if xcd_shared else 0))
         self.cfg = Config(team_waves, plain_heaps, max_rounds, hash_log2, mem_budget, puzzle_team_waves, flags)
         h = C.c_void_p()
         rc = lib().ua_create(ordinal, C.byref(self.cfg), C.byref(h))

@@ -169,13 +169,23 @@ enum : int { CLS_PUZZLE = 0, CLS_GRID = 1, CLS_GRID_LARGE = 2, CLS_WIDE = 3 };
 // in order of c in block order); its groups pull the queries qs[qoff[c] .. qoff[c] +
 // nq[c]) from counter qhead[c]. A group's insert buffer holds segcap[c] keys per
 // round half.
+//
+// XCD split (iso_xcds > 0): the device's kXcds XCDs each have their own L2, and a
+// workgroup b of a launch runs on XCD b mod kXcds. The classes of iso_mask (one-block
+// teams, whose barriers never write back or invalidate an L2) then run on the last
+// iso_xcds XCDs and every other class (teams of several blocks, whose agent-scope
+// barriers do, and which stream pattern databases and hashes through the L2) on the
+// first kXcds - iso_xcds; each side's classes fill its XCDs in class order, interleaved
+// across them. Class order still fixes every group's index and insert buffer.
+constexpr int kXcds = 8;
 struct LaunchShape {
     int32_t groups[kClasses];
     int32_t bpg[kClasses];
     int32_t nq[kClasses];
     int32_t qoff[kClasses];
     int32_t segcap[kClasses];
-    int32_t pad;
+    int16_t iso_mask;  // bit c: class c runs on the split-off XCDs
+    int16_t iso_xcds;  // XCDs split off (0 = every class on every XCD)
     // serving: each group stops starting queries, and suspends the one it runs, once it
     // has been busy this long in the launch (s_memrealtime ticks, 100 MHz; 0 = no limit)
     int64_t budget_ticks;

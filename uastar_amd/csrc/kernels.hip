@@ -21,7 +21,7 @@ namespace ua {
 template <int MODE, int SET>
 __global__ __launch_bounds__(1024) void k_teams(char *io, uint64_t *segs) {
     StepHeader *h = reinterpret_cast<StepHeader *>(io);
-    if ((int)blockIdx.x >= shape_blocks(h->shape)) return;
+    if (h->shape.iso_xcds == 0 && (int)blockIdx.x >= shape_blocks(h->shape)) return;
     group_block<SET>(reinterpret_cast<const QueryDesc *>(io + h->qd_off),
                      reinterpret_cast<QueryResult *>(io + h->qr_off), h->shape, h->qhead,
                      reinterpret_cast<TeamVars *>(io + h->tv_off), segs);

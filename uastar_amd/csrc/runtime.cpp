@@ -211,6 +211,7 @@ void fill_result(const QueryDesc &q, const QueryResult &r, ua_result *o) {
             o->cost = (double)o->steps;
         }
     }
+    o->heaps = q.heaps;
     o->rounds = r.rounds;
     o->pops = r.pops;
     o->expanded = r.expanded;
@@ -756,6 +757,49 @@ int team_waves(ua_ctx *c, int kind, int64_t cells) {
     return blocks * kBlockWaves;
 }
 
+// XCD split of a batch or serving launch (see LaunchShape): the one-block classes get XCDs
+// of their own, as many as their blocks need (at least one, and at least one is left to the
+// other classes). Then each side's classes grow into their side's spare blocks (up to their
+// query counts), or shrink, largest first and down to one group, until they fit it. The
+// shape is left unsplit when one side is empty, the device's blocks are not a multiple of
+// kXcds, or a side cannot fit.
+static void split_xcds(LaunchShape &shape, int resident) {
+    if (resident % kXcds) return;
+    const int per = resident / kXcds;
+    LaunchShape s = shape;
+    int mask = 0, blocks[2] = {0, 0};  // [0] the other classes, [1] the split-off ones
+    for (int k = 0; k < kClasses; ++k) {
+        if (!s.nq[k]) continue;
+        const int side = s.bpg[k] == 1;
+        if (side) mask |= 1 << k;
+        blocks[side] += s.groups[k] * s.bpg[k];
+    }
+    if (!blocks[0] || !blocks[1]) return;
+    const int iso = std::min(kXcds - 1, std::max(1, (blocks[1] + per - 1) / per));
+    const int cap[2] = {(kXcds - iso) * per, iso * per};
+    for (int side = 0; side < 2; ++side) {
+        auto on_side = [&](int k) { return s.nq[k] && (((mask >> k) & 1) == side); };
+        while (blocks[side] > cap[side]) {
+            int big = -1;
+            for (int k = 0; k < kClasses; ++k)
+                if (on_side(k) && s.groups[k] > 1 &&
+                    (big < 0 || s.groups[k] * s.bpg[k] > s.groups[big] * s.bpg[big]))
+                    big = k;
+            if (big < 0) return;
+            --s.groups[big];
+            blocks[side] -= s.bpg[big];
+        }
+        for (int k = 0; k < kClasses; ++k)
+            while (on_side(k) && s.groups[k] < s.nq[k] && blocks[side] + s.bpg[k] <= cap[side]) {
+                ++s.groups[k];
+                blocks[side] += s.bpg[k];
+            }
+    }
+    s.iso_mask = (int16_t)mask;
+    s.iso_xcds = (int16_t)iso;
+    shape = s;
+}
+
 // Runs the planned queries of one problem set in one cooperative launch of k_teams.
 // Each query's class (CLS_*) gets groups of heaps / kBlockWaves blocks; the resident
 // blocks are shared among the classes in proportion to their demand (queries x blocks
@@ -812,6 +856,7 @@ static int run_set(ua_ctx *c, int set, std::vector<QueryDesc> &descs, std::vecto
         --shape.groups[big];
         used -= shape.bpg[big];
     }
+    if (mode != 0 && !(c->cfg.flags & UA_FLAG_XCD_SHARED)) split_xcds(shape, resident);
     shape.budget_ticks = budget_ticks;
     std::vector<QueryDesc> sorted(nq);
     for (int k = 0; k < nq; ++k) sorted[k] = descs[order[k]];

@@ -59,11 +59,12 @@ struct Scope {
         return wg ? __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
                   : __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    // A read that may return an older value within the round (workgroup scope: served
-    // by this CU's caches, which the team barrier's acquire refreshes). For the
-    // monotone search state only -- best words and hash values only decrease, a hash
-    // key changes once from 0 -- where an older value costs at most a failing atomic or
-    // an expansion with a cost that is no longer the best, never a wrong answer.
+    // A read that may return an older value (workgroup scope: served by this CU's
+    // caches, which nothing refreshes between the rounds of a team of several blocks).
+    // For the monotone search state only -- best words and hash values only decrease, a
+    // hash key changes once from 0 -- where an older value costs at most a failing or
+    // needless atomic, never a wrong answer. The state of a popped key is read with
+    // load(), whose value is at least as new as the one that pushed the key.
     template <class T>
     __device__ __forceinline__ T load_weak(const T *p) const {
         return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -172,12 +173,18 @@ struct GroupTeam {
     // block into an abort that every block of the group observes, never a hang.
     // A one-block group is one workgroup barrier around the hook.
     //
-    // Across blocks, agent-scope ordering costs an L2 write-back (release) or an L1/L2
-    // invalidation (acquire) on this XCD, which also evicts the data of the other
-    // blocks there. So only thread 0 orders, once each way per barrier, around relaxed
-    // atomics (a release fence before the arrival, an acquire fence after the
-    // generation moves); the workgroup barriers carry the block's other threads. The
-    // spin itself reads relaxed.
+    // Across blocks, an agent-scope fence would cost an L2 write-back (release) or an
+    // L1/L2 invalidation (acquire) on this XCD, which also evicts the data of the other
+    // blocks there. So a group of several blocks exchanges everything its blocks share
+    // -- insert buffer, round slots, hash, best words, counters -- only through
+    // agent-scope atomics (relaxed loads and stores at agent scope go to the device's
+    // coherence point, past this XCD's L2), and the barrier orders them without any
+    // agent-scope fence: every wave waits until its own memory operations are complete
+    // (s_waitcnt 0) before the workgroup barrier and thread 0's arrival, and every read
+    // of shared data after the barrier is an agent-scope load issued after thread 0 saw
+    // the generation move. The workgroup fences keep the compiler from moving memory
+    // operations across the barrier. Per-wave heaps stay private to their wave (one CU,
+    // one L2) and are never exchanged.
     template <class Hook>
     __device__ __forceinline__ bool sync(Hook hook) const {
         if (bpg == 1) {
@@ -191,17 +198,19 @@ struct GroupTeam {
         }
         constexpr int SC = __HIP_MEMORY_SCOPE_AGENT;
         __shared__ uint32_t s_ok;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_s_waitcnt(0);  // this wave's stores and atomics are complete
         __syncthreads();
         if (threadIdx.x == 0) {
             uint32_t ok = 1;
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // the block's writes, before arriving
             uint32_t g = __hip_atomic_load(&tv->bar_gen, __ATOMIC_RELAXED, SC);
             uint32_t a = __hip_atomic_fetch_add(&tv->bar_count, 1u, __ATOMIC_RELAXED, SC);
             if (a == (uint32_t)bpg - 1) {
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // the others' writes, for the hook
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
                 hook();
                 __hip_atomic_store(&tv->bar_count, 0u, __ATOMIC_RELAXED, SC);
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // the hook's writes
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                __builtin_amdgcn_s_waitcnt(0);  // the hook's writes are complete
                 __hip_atomic_fetch_add(&tv->bar_gen, 1u, __ATOMIC_RELAXED, SC);
             } else {
                 uint32_t spins = 0;
@@ -215,10 +224,11 @@ struct GroupTeam {
                 }
             }
             if (__hip_atomic_load(&tv->abort, __ATOMIC_RELAXED, SC)) ok = 0;
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // everyone's writes, for the whole block
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
             s_ok = ok;
         }
         __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
         return s_ok != 0;
     }
 };
@@ -270,11 +280,15 @@ struct GridProb {
     // accesses: the cell's best word; every in-range neighbour's best word; an atomicMin
     // for each open neighbour (or the goal) the move would improve. Coordinates and
     // offsets are 32-bit; a neighbour's word is addressed from the cell's (rows +-W).
-    // *gk_out receives the cell's g key when the cell was expanded.
+    // *gk_out receives the cell's g key when the cell was expanded. A key whose exact f
+    // (the cell's g key plus its octile key) is at least goal_bound, the best goal found
+    // so far, is dropped unexpanded: with a consistent h no descendant reaches a cheaper
+    // goal. (The key's own f field is rounded down, so only this exact test can drop the
+    // keys of the goal's own f.)
     template <class SC>
     __device__ __forceinline__ void expand(const SC &sc, uint64_t it, uint64_t (&s)[MAXS], int &cnt,
                                            uint64_t &goalmin, uint32_t &err, Ctr &k,
-                                           uint64_t *gk_out = nullptr) const {
+                                           uint64_t *gk_out = nullptr, uint64_t goal_bound = SENT) const {
         const uint64_t cell = it & cmask;
         const int x = (int)(cell >> q.ybits), y = (int)(cell & ymask);
         if (x >= q.H || y >= q.W) {  // never produced by a correct round: fail the query, touch nothing
@@ -282,9 +296,11 @@ struct GridProb {
             return;
         }
         UA_GLOBAL uint64_t *const bp = gptr(q.best) + ((uint64_t)(uint32_t)x * (uint32_t)q.W + (uint32_t)y);
-        const uint64_t bw = sc.load_weak(bp);
+        const uint64_t bw = sc.load(bp);  // the stale test needs the word that pushed the key, or a lower one
         const uint64_t gk = bw >> kGridKeyShift;
-        if (((gk + oct(uabs(x - q.ex), uabs(y - q.ey))) >> q.qshift) != (it >> q.cbits)) return;  // stale
+        const uint64_t fk = gk + oct(uabs(x - q.ex), uabs(y - q.ey));
+        if ((fk >> q.qshift) != (it >> q.cbits)) return;  // stale
+        if (fk >= goal_bound) return;                     // cannot lead to a cheaper goal
         ++k.exp;
         if (!(bw & kOpenBit)) return;  // source-cell mask rule: a blocked cell has no moves
         if (gk_out) *gk_out = gk;
@@ -563,8 +579,10 @@ struct PuzzleProb {
             return;
         }
         uint32_t gi = 255u - (uint32_t)((it >> 48) & 255u);
-        uint64_t b = sc.load_weak(gptr(q.best) + slot);
-        uint32_t v = sc.load_weak(gptr(q.vals) + slot);
+        // the popped state's board and value may have been written from another block:
+        // scoped loads (the stale test must see the value that pushed the key, or a lower one)
+        uint64_t b = sc.load(gptr(q.best) + slot);
+        uint32_t v = sc.load(gptr(q.vals) + slot);
         uint32_t gb = v >> 4;
         if (gb != gi) return;  // stale
         ++k.exp;
@@ -757,9 +775,10 @@ struct PuzzleProb5 {
         }
         uint32_t gi = 255u - (uint32_t)((it >> 48) & 255u);
         int hp = (int)((it >> 40) & 255u);
-        uint64_t w0 = sc.load_weak(gptr(q.best) + (2 * slot));
-        uint64_t w1 = sc.load_weak(gptr(q.best) + (2 * slot + 1));
-        uint32_t v = sc.load_weak(gptr(q.vals) + (slot));
+        // written from any block of the team: scoped loads (see PuzzleProb::expand)
+        uint64_t w0 = sc.load(gptr(q.best) + (2 * slot));
+        uint64_t w1 = sc.load(gptr(q.best) + (2 * slot + 1));
+        uint32_t v = sc.load(gptr(q.vals) + (slot));
         uint32_t gb = v >> 4;
         if (gb != gi) return;  // stale
         ++k.exp;
@@ -893,11 +912,11 @@ __device__ __forceinline__ void team_solve(const QueryDesc &q, QueryResult *res,
             uint32_t base = 0;
             if (l == 0 && total) base = sc.fadd(&cur->cnt, (uint32_t)total);
             base = __shfl(base, 0);
-            uint64_t *out = cur_seg + base + off;
+            uint64_t *out = cur_seg + base + off;  // shared by the team's blocks: scoped stores
 #pragma unroll
             for (int i = 0; i < Prob::MAXS; ++i)
-                if (s[i] != SENT) *out++ = s[i];
-            if (keep != SENT) *out = keep;
+                if (s[i] != SENT) sc.store(out++, s[i]);
+            if (keep != SENT) sc.store(out, keep);
         };
         if constexpr (Prob::kContinue) {
             // Only a wave whose heap still holds keys after its pop continues: its pops were
@@ -979,7 +998,7 @@ __device__ __forceinline__ void team_solve(const QueryDesc &q, QueryResult *res,
         bool fits = true;
         for (int b0 = 0; b0 < mine; b0 += 64) {
             int k = b0 + l;
-            uint64_t v = k < mine ? cur_seg[(size_t)k * W + rot] : SENT;
+            uint64_t v = k < mine ? sc.load(cur_seg + ((size_t)k * W + rot)) : SENT;
             int sreal = mine - b0 < 64 ? mine - b0 : 64;
             v = wave_sort(v);
             fits &= heap.insert(v, sreal);
@@ -1134,7 +1153,7 @@ __device__ __forceinline__ void team_solve_buckets(const QueryDesc &q, QueryResu
     if (!resumed && team.leader()) {  // the start's record and key
         const uint64_t key0 = P.seed(sc);
         const uint32_t ring = (uint32_t)cur & RM;
-        store[(uint64_t)ring << blog] = key0;
+        sc.store(store + ((uint64_t)ring << blog), key0);
         sc.store(&bm->tail[ring], 1u);
     }
     if (!team.sync([] {})) return;
@@ -1260,7 +1279,7 @@ __device__ __forceinline__ void team_solve_buckets(const QueryDesc &q, QueryResu
         uint64_t it = SENT;
         if (j < total) {
             const uint32_t ring = ((uint32_t)cur + 4u * (uint32_t)L + (uint32_t)e) & RM;
-            it = store[((uint64_t)ring << blog) | ((hsel + off) & bmask)];
+            it = sc.load(store + (((uint64_t)ring << blog) | ((hsel + off) & bmask)));
         }
         const uint32_t cur0 = (uint32_t)cur;  // the snapshot's first ring bucket
         cur = blo;
@@ -1276,7 +1295,7 @@ __device__ __forceinline__ void team_solve_buckets(const QueryDesc &q, QueryResu
             ++n_pop;
             if (P.dominated(it, best_goal)) it = SENT;
         }
-        if (it != SENT) P.expand(sc, it, s, c, goalmin, err, k, &gk);
+        if (it != SENT) P.expand(sc, it, s, c, goalmin, err, k, &gk, best_goal);
         // continuation only in a saturated round (the window held more keys than the round
         // takes), so that the chains run ahead of a best-first frontier
         uint64_t keep = SENT;
@@ -1332,7 +1351,7 @@ __device__ __forceinline__ void team_solve_buckets(const QueryDesc &q, QueryResu
                 err |= FLAG_CAPACITY;
                 continue;
             }
-            store[((uint64_t)ring << blog) | (pos[i] & bmask)] = key;
+            sc.store(store + (((uint64_t)ring << blog) | (pos[i] & bmask)), key);
         }
         if (__ballot(goalmin != SENT)) {
             goalmin = wave_min64(goalmin);
@@ -1484,12 +1503,34 @@ template <int SET = SET_MAIN>
 __device__ __forceinline__ void group_block(const QueryDesc *qs, QueryResult *rs, const LaunchShape &shape,
                                             uint32_t *qhead, TeamVars *tvs, uint64_t *segs) {
     int rel = (int)blockIdx.x, cls = 0, group = 0;
+    if (shape.iso_xcds > 0) {
+        // XCD split: the block's side (XCD blockIdx mod kXcds), its index among that side's
+        // blocks (interleaved across the side's XCDs), then its class among the side's
+        const int x = (int)blockIdx.x % kXcds, s = (int)blockIdx.x / kXcds;
+        const int P = kXcds - shape.iso_xcds;
+        const bool split_off = x >= P;
+        rel = split_off ? s * shape.iso_xcds + (x - P) : s * P + x;
+        cls = -1;
+        for (int k = 0; k < kClasses; ++k) {
+            if ((((shape.iso_mask >> k) & 1) != 0) != split_off) continue;
+            const int n = shape.groups[k] * shape.bpg[k];
+            if (rel < n) {
+                cls = k;
+                break;
+            }
+            rel -= n;
+        }
+        if (cls < 0) return;  // past its side's teams: idle
+    } else {
+        while (cls + 1 < kClasses && rel >= shape.groups[cls] * shape.bpg[cls]) {
+            rel -= shape.groups[cls] * shape.bpg[cls];
+            ++cls;
+        }
+    }
     size_t seg_off = 0;
-    while (cls + 1 < kClasses && rel >= shape.groups[cls] * shape.bpg[cls]) {
-        rel -= shape.groups[cls] * shape.bpg[cls];
-        group += shape.groups[cls];
-        seg_off += (size_t)shape.groups[cls] * 2 * (size_t)shape.segcap[cls];
-        ++cls;
+    for (int k = 0; k < cls; ++k) {
+        group += shape.groups[k];
+        seg_off += (size_t)shape.groups[k] * 2 * (size_t)shape.segcap[k];
     }
     const int bpg = shape.bpg[cls];
     group += rel / bpg;

@@ -80,7 +80,8 @@ extern "C" int ua_set_tuning(ua_ctx *c, const ua_tuning *t) {
     auto in = [](int64_t v, int64_t lo, int64_t hi) { return v >= lo && v <= hi; };
     if (!width(t->grid_team_waves) || !width(t->grid_large_team_waves) || !width(t->puzzle_team_waves) ||
         !in(t->grid_large_cells, 1, 1ll << 40) || !in(t->puzzle_hash_log2, 10, 32) ||
-        !in(t->plain_grid_heaps, 4, 1 << 20) || !in(t->plain_puzzle_heaps, 4, 1 << 20) ||
+        (t->plain_grid_heaps && !in(t->plain_grid_heaps, 4, 1 << 20)) ||
+        (t->plain_puzzle_heaps && !in(t->plain_puzzle_heaps, 4, 1 << 20)) ||
         !in(t->plain_hash_log2, 10, 32))
         return fail(c, UA_ERR_INVALID, "tuning value out of range (team widths are multiples of 16 waves)");
     c->tune.grid_waves = t->grid_team_waves;
