@@ -8,7 +8,9 @@
 //      time budget, then suspends the query it runs (heap tops and counters go to the
 //      query's resume record; the state stays resident in HBM). A query the launch does
 //      not reach waits for the next one;
-//   3. retires the queries that finished (found / no path / error), frees their memory.
+//   3. retires the queries that finished (found / no path / error), frees their memory;
+//   4. restarts a 15-puzzle that has run long on a wide team (a quarter of the device)
+//      with a larger hash.
 // Queries that arrive while others are on the device are admitted at the next step,
 // so their work overlaps on the device with the queries already in flight. A heavy
 // query runs for the whole launch on its team; light ones finish and make room. A query
@@ -78,8 +80,7 @@ struct Job {
     QueryDesc d{};
     Plan plan;
     size_t off = 0, bytes = 0;    // pool region: best table / hash, then heaps and resume record
-    size_t off2 = 0, bytes2 = 0;  // widened: heaps and resume record of the wide team
-    bool wide = false;
+    bool wide = false;  // planned for a wide team (CLS_WIDE) with a larger hash
     int scale = 1;      // grid heap scale / puzzle hash growth steps
     int launches = 0;
     int64_t rounds = 0;  // rounds completed so far
@@ -98,8 +99,9 @@ struct ServeEngine {
     Pool pool;
     bool pool_ready = false;
     int64_t budget_ticks = 400000;  // device time per launch: 4 ms of s_memrealtime (100 MHz)
-    double widen_after_ms = 25.0;   // a puzzle that has run this long moves to a wide team
+    double widen_after_ms = 25.0;   // a puzzle that has run this long restarts on a wide team
     int max_wide = 2;               // widened queries at a time (each takes a quarter of the device)
+    int wide_hash_grow = 4;         // a widened puzzle's hash has 2^4 times the slots
     int64_t widened = 0;
     int64_t launches = 0;
     double kernel_ms = 0;
@@ -111,6 +113,13 @@ static int64_t cells_of(const ua_query &q) {
 }
 static int cls_of_query(ua_ctx *c, const ua_query &q) {
     return class_of(c, q.kind == UA_QUERY_PUZZLE ? KIND_PUZZLE : KIND_GRID, cells_of(q));
+}
+static int cls_of_job(ua_ctx *c, const Job &j) { return j.wide ? CLS_WIDE : cls_of_query(c, j.q); }
+// heaps of a widened query's team: a quarter of the device's resident waves
+static int wide_waves(ua_ctx *c) {
+    int per_cu = 0;
+    if (teams_max_blocks(&per_cu) != hipSuccess || per_cu < 1) per_cu = 1;
+    return kBlockWaves * std::max(1, c->num_cu * per_cu / 4);
 }
 
 void ua_serve_release(ua_ctx *c) {
@@ -131,7 +140,8 @@ ServeEngine *engine(ua_ctx *c) {
 // Plans job j for its current scale and sets its device pointers inside the pool.
 int plan_job(ua_ctx *c, ServeEngine *e, Job &j, bool *fits) {
     *fits = false;
-    const int heaps = j.q.kind == UA_QUERY_PUZZLE
+    const int heaps = j.wide ? wide_waves(c)
+                      : j.q.kind == UA_QUERY_PUZZLE
                           ? team_waves(c, KIND_PUZZLE, 0)
                           : team_waves(c, KIND_GRID, j.q.grid ? (int64_t)j.q.grid->H * j.q.grid->W : 0);
     int rc;
@@ -141,9 +151,10 @@ int plan_job(ua_ctx *c, ServeEngine *e, Job &j, bool *fits) {
         rc = ensure_pdb(c, j.q.n);
         if (!rc) {
             int base = batch_hash_log2(c, j.q.n);
-            int grow = 0;
+            int grow = j.wide ? e->wide_hash_grow : 0;
             for (int s = j.scale; s > 1; s /= 4) grow += 2;
-            rc = puzzle_fill_desc(c, j.q.n, j.q.tiles, heaps, base + grow, j.d, j.plan);
+            rc = puzzle_fill_desc(c, j.q.n, j.q.tiles, heaps, std::min(32, base + grow), j.d, j.plan);
+            if (j.wide) j.d.cls = CLS_WIDE;
         }
     }
     if (rc) return rc;
@@ -179,47 +190,6 @@ int plan_job(ua_ctx *c, ServeEngine *e, Job &j, bool *fits) {
         *fits = false;
         HIPCHK(c, he);
     }
-    return UA_OK;
-}
-
-// Moves a long-running served puzzle to a wide team of `waves` heaps. Its heaps keep
-// their contents as the first heaps of the wider team, the new heaps start empty and
-// fill from the rounds' insert distribution; its hash table stays where it is. Runs
-// between launches. Without pool memory for the wider heaps the query stays as it is.
-int widen(ua_ctx *c, ServeEngine *e, Job &j, int waves) {
-    const int w0 = j.d.heaps;
-    const size_t per_heap = (size_t)(j.d.heap_cap - 1) * 64 * 8;
-    const size_t heap_bytes = (size_t)waves * per_heap;
-    const size_t bytes2 = align256(heap_bytes) + align256((size_t)resume_words(waves) * 8);
-    size_t off2;
-    if (!e->pool.alloc(bytes2, &off2)) return UA_OK;
-    char *base2 = e->pool.arena.as<char>() + off2;
-    uint64_t *heap2 = reinterpret_cast<uint64_t *>(base2);
-    uint64_t *rec2 = reinterpret_cast<uint64_t *>(base2 + align256(heap_bytes));
-    // the new heaps: empty root and partial batches (all ones), n = pc = 0
-    std::vector<uint64_t> fresh((size_t)(waves - w0) * RESUME_PER_HEAP, ~0ull);
-    for (int h = 0; h < waves - w0; ++h) fresh[(size_t)h * RESUME_PER_HEAP + 128] = fresh[(size_t)h * RESUME_PER_HEAP + 129] = 0;
-    hipError_t he = hipMemcpyAsync(heap2, j.d.heap, (size_t)w0 * per_heap, hipMemcpyDeviceToDevice, c->st);
-    if (he == hipSuccess)
-        he = hipMemcpyAsync(rec2, j.d.resume, (size_t)resume_words(w0) * 8, hipMemcpyDeviceToDevice, c->st);
-    if (he == hipSuccess)
-        he = hipMemcpyAsync(rec2 + resume_words(w0), fresh.data(), fresh.size() * 8, hipMemcpyHostToDevice, c->st);
-    if (he == hipSuccess) he = hipStreamSynchronize(c->st);
-    if (he != hipSuccess) {  // the query keeps its narrow team; give the wide region back
-        e->pool.release(off2, bytes2);
-        HIPCHK(c, he);
-    }
-    const size_t keep = align256(j.plan.best_bytes) + align256(j.plan.vals_bytes);
-    e->pool.release(j.off + keep, j.bytes - keep);  // the narrow team's heaps and record
-    j.bytes = keep;
-    j.off2 = off2;
-    j.bytes2 = bytes2;
-    j.plan.heap_bytes = heap_bytes;
-    j.d.heaps = waves;
-    j.d.heap = heap2;
-    j.d.resume = rec2;
-    j.d.cls = CLS_WIDE;
-    j.wide = true;
     return UA_OK;
 }
 
@@ -277,7 +247,8 @@ extern "C" int ua_serve_step(ua_ctx *c, int64_t *n_active, int64_t *n_pending) {
         const int64_t cells = k == CLS_GRID_LARGE ? c->tune.grid_large_cells : 0;
         cap_cls[k] = (size_t)std::max(1, 2 * resident / (team_waves(c, kind, cells) / kBlockWaves));
     }
-    for (const Job &j : e->active) n_cls[cls_of_query(c, j.q)]++;
+    cap_cls[CLS_WIDE] = (size_t)e->max_wide;
+    for (const Job &j : e->active) n_cls[cls_of_job(c, j)]++;
 
     // 1. admission (in arrival order; a class at its cap waits, the other can still be admitted)
     {
@@ -290,7 +261,7 @@ extern "C" int ua_serve_step(ua_ctx *c, int64_t *n_active, int64_t *n_pending) {
                 j = e->pending.front();
                 e->pending.pop_front();
             }
-            int k = cls_of_query(c, j.q);
+            int k = cls_of_job(c, j);
             if (n_cls[k] >= cap_cls[k]) {
                 later.push_back(j);
                 continue;
@@ -348,9 +319,6 @@ extern "C" int ua_serve_step(ua_ctx *c, int64_t *n_active, int64_t *n_pending) {
             }
             auto give_back = [&] {
                 e->pool.release(j.off, j.bytes);
-                if (j.bytes2) e->pool.release(j.off2, j.bytes2);
-                j.bytes2 = 0;
-                j.wide = false;
             };
             if (r.status == ST_CAPACITY && j.scale < 256) {  // restart with a larger plan
                 give_back();
@@ -381,18 +349,38 @@ extern "C" int ua_serve_step(ua_ctx *c, int64_t *n_active, int64_t *n_pending) {
             complete(e, j, out);
         }
         e->active.swap(still);
-        // 4. widen long-running puzzles (frontier-bound: more heaps pop more per round)
-        const int wide_waves = kBlockWaves * std::max(1, resident / 4);
+        // 4. widen long-running puzzles (frontier-bound: more heaps pop more per round). A
+        // puzzle that has run widen_after_ms restarts from its start board on a team of a
+        // quarter of the device with a hash of 2^wide_hash_grow times the slots: a long
+        // search on a wide team generates states several times faster, and a hash near
+        // full lengthens every probe and ends in a capacity restart anyway. The narrow
+        // team's work is lost, about an eighth of what the wide team does in the same time.
+        const int ww = wide_waves(c);
         int n_wide = 0;
         for (const Job &j : e->active) n_wide += j.wide;
+        {
+            std::lock_guard<std::mutex> g(e->mu);
+            for (const Job &j : e->pending) n_wide += j.wide;
+        }
+        std::vector<Job> keep;
+        std::vector<Job> restart;
         for (Job &j : e->active) {
-            if (n_wide >= e->max_wide) break;
-            if (j.wide || j.q.kind != UA_QUERY_PUZZLE || j.device_ms < e->widen_after_ms || j.d.heaps >= wide_waves)
-                continue;
-            int wrc = widen(c, e, j, wide_waves);
-            if (wrc) return wrc;
-            n_wide += j.wide;
-            e->widened += j.wide;
+            if (n_wide < e->max_wide && !j.wide && j.q.kind == UA_QUERY_PUZZLE && j.q.n == 4 &&
+                j.device_ms >= e->widen_after_ms && j.d.heaps < ww) {
+                e->pool.release(j.off, j.bytes);
+                j.wide = true;
+                j.rounds = 0;
+                ++n_wide;
+                ++e->widened;
+                restart.push_back(j);
+            } else {
+                keep.push_back(j);
+            }
+        }
+        e->active.swap(keep);
+        if (!restart.empty()) {
+            std::lock_guard<std::mutex> g(e->mu);
+            for (auto it = restart.rbegin(); it != restart.rend(); ++it) e->pending.push_front(*it);
         }
     }
     if (n_active) *n_active = (int64_t)e->active.size();
