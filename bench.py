@@ -449,6 +449,10 @@ def run_serve_mixed(args):
         "atomics": {"total": atomics, "per_launch": atomics / max(1, launches),
                     "per_s": atomics / (kms * 1e-3) if kms > 0 else 0.0},
     }
+    # the device's random-atomic throughput (ua_debug_atomic_rate, after the timed passes),
+    # the ceiling of the relaxations
+    peak = ctx.atomic_rate() * 1e9
+    out["atomics"].update({"probe_peak_per_s": peak, "frac": out["atomics"]["per_s"] / peak if peak else None})
     def diag(res, lat=None):
         rows = []
         for k, (r, sp) in enumerate(zip(res, specs)):
@@ -556,11 +560,18 @@ def main():
                     help="grids keep one sorted heap per wave instead of the team's bucket ring (comparison)")
     ap.add_argument("--xcd-shared", action="store_true",
                     help="serving: every team class on every XCD (no XCDs apart for one-block grid teams)")
+    ap.add_argument("--atomic-probe", action="store_true",
+                    help="only measure the device's random 32-bit atomicMin throughput (ua_debug_atomic_rate)")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (rocprofv3 counter calibration)")
     args = ap.parse_args()
     if args.copy_calib:
         print(json.dumps(copy_calib(args.copy_calib)))
+        return
+    if args.atomic_probe:
+        import uastar_amd as U
+        with U.Context(0) as ctx:
+            print(json.dumps({f"blocks_{b}_gatomics_per_s": ctx.atomic_rate(26, b, 64) for b in (256, 1024, 4096)}))
         return
     if args.gpus != 1:
         print("bench.py runs on one GPU: every query is solved on a single MI355X "

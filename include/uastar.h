@@ -231,6 +231,10 @@ int ua_get_tuning(ua_ctx *ctx, ua_tuning *out);
 int ua_set_tuning(ua_ctx *ctx, const ua_tuning *t);
 
 /* ---- diagnostics */
+/* Measures random device-memory atomic throughput: blocks x 256 threads each issue iters
+ * 32-bit atomicMins (agent scope) at hashed positions of a fresh array of 2^log2_words
+ * words; *gops = atomics per nanosecond. The roofline of the search's relaxations. */
+int ua_debug_atomic_rate(ua_ctx *ctx, int log2_words, int blocks, int iters, double *gops);
 /* Runs a random insert/pop sequence on one wave's heap; see tests. */
 int ua_debug_heap_selftest(ua_ctx *ctx, const int32_t *ops, int nops, const uint64_t *keys, int64_t nkeys,
                            int cap_nodes, uint64_t *out_pops, int64_t out_cap, int32_t *final_size);

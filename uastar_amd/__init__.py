@@ -120,6 +120,7 @@ def lib():
             "ua_launch_stats": (C.c_int, [vp, C.POINTER(i64), C.POINTER(i64), C.POINTER(C.c_double),
                                           C.POINTER(C.c_double)]),
             "ua_launch_graph_error": (C.c_char_p, [vp]),
+            "ua_debug_atomic_rate": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]),
             "ua_debug_heap_selftest": (C.c_int, [vp, vp, C.c_int, vp, i64, C.c_int, vp, i64,
                                                  C.POINTER(i32)]),
             "ua_autotune": (C.c_int, [vp, C.POINTER(Query), i64, C.POINTER(Tuning)]),
@@ -402,6 +403,12 @@ class Context:
         self._check(lib().ua_set_tuning(self.h, C.byref(t)))
 
     # ---- diagnostics
+    def atomic_rate(self, log2_words=26, blocks=4096, iters=64) -> float:
+        """Random agent-scope atomicMin throughput of the device, in G atomics/s."""
+        g = C.c_double()
+        self._check(lib().ua_debug_atomic_rate(self.h, log2_words, blocks, iters, C.byref(g)))
+        return g.value
+
     def heap_selftest(self, ops, keys, cap_nodes):
         ops = np.ascontiguousarray(ops, dtype=np.int32)
         keys = np.ascontiguousarray(keys, dtype=np.uint64)

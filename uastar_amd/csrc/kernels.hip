@@ -271,8 +271,27 @@ __global__ void k_heap_selftest(const int *ops, int nops, const uint64_t *keys, 
     if (l == 0) *status = ok ? h.size() : -1;
 }
 
+// Throughput probe of random device-memory atomics: every thread issues `iters`
+// atomicMins at agent scope on 32-bit words of a[0 .. mask] picked by a hash of (thread,
+// i), the access pattern of the search's relaxations. The sum of the old values goes to
+// *sink only if it hits a magic value, so that no result is dead.
+__global__ __launch_bounds__(256) void k_atomic_probe(uint32_t *a, uint64_t mask, int iters, uint32_t *sink) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t acc = 0;
+    for (int i = 0; i < iters; ++i) {
+        const uint64_t idx = mix64(t * 0x9e3779b97f4a7c15ull + (uint64_t)i) & mask;
+        acc += __hip_atomic_fetch_min(a + idx, (uint32_t)(t + (uint64_t)i), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (acc == 0x12345678u) sink[0] = acc;
+}
+
 // ------------------------------------------------------------------ launchers
 static int grid_blocks(size_t n, int t) { return (int)((n + t - 1) / t); }
+
+hipError_t launch_atomic_probe(uint32_t *a, uint64_t mask, int blocks, int iters, uint32_t *sink, hipStream_t st) {
+    hipLaunchKernelGGL(k_atomic_probe, dim3(blocks), dim3(256), 0, st, a, mask, iters, sink);
+    return hipGetLastError();
+}
 
 hipError_t launch_grid_gen(uint64_t *bm, int H, int W, int wpr, uint64_t seed, uint64_t thr, int sx,
                            int sy, int ex, int ey, hipStream_t st) {

@@ -1121,3 +1121,27 @@ extern "C" int ua_debug_heap_selftest(ua_ctx *c, const int32_t *ops, int nops, c
     for (DevBuf *b : {&d_ops, &d_keys, &d_store, &d_out, &d_st}) b->release();
     return UA_OK;
 }
+
+// Random device-memory atomic throughput (k_atomic_probe): blocks x 256 threads x iters
+// atomicMins over 2^log2_words 32-bit words, timed with HIP events; *gops = atomics / ns.
+extern "C" int ua_debug_atomic_rate(ua_ctx *c, int log2_words, int blocks, int iters, double *gops) {
+    if (!c || !gops || log2_words < 10 || log2_words > 32 || blocks < 1 || iters < 1)
+        return fail(c, UA_ERR_INVALID, "bad arguments");
+    const uint64_t n = 1ull << log2_words;
+    DevBuf a, sink;
+    HIPCHK(c, a.ensure(n * 4));
+    HIPCHK(c, sink.ensure(64));
+    HIPCHK(c, hipMemsetAsync(a.p, 0xFF, n * 4, c->st));
+    float ms = 0;
+    for (int rep = 0; rep < 2; ++rep) {  // the first launch warms up; the second is timed
+        HIPCHK(c, hipEventRecord(c->ev0, c->st));
+        HIPCHK(c, launch_atomic_probe(a.as<uint32_t>(), n - 1, blocks, iters, sink.as<uint32_t>(), c->st));
+        HIPCHK(c, hipEventRecord(c->ev1, c->st));
+        HIPCHK(c, hipStreamSynchronize(c->st));
+        HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    }
+    a.release();
+    sink.release();
+    *gops = (double)blocks * 256.0 * iters / ((double)ms * 1e6);
+    return UA_OK;
+}
