@@ -493,7 +493,7 @@ def run_deep_grid(args):
     query), `steps` times after `warmup` small grids. Its rounds are set by the path's
     depth, so the time per round is the figure that bounds the serving stream's tail."""
     import uastar_amd as U
-    ctx = U.Context(0, grid_heaps=args.grid_heaps)
+    ctx = U.Context(0, grid_heaps=args.grid_heaps, team_waves=args.team_waves)
     specs = [s for s in mixed_stream(204) if s["kind"] == "grid"]
     deep = max(specs, key=lambda s: max(abs(s["s"][0] - s["g"][0]), abs(s["s"][1] - s["g"][1])))
     warm = [s for s in specs if s["H"] < 1000][: args.warmup]
@@ -525,7 +525,7 @@ def run_deep_grid(args):
         "data": "synthetic (seeded generators)",
         "config": {"workload": f"the serving stream's deepest grid alone: {deep['H']}x{deep['W']}, "
                                f"p = {deep['p']:.3f}, start {tuple(deep['s'])}, goal {tuple(deep['g'])}",
-                   "mode": "batch of one query (one one-block team)"},
+                   "mode": f"batch of one query (a team of {r['heaps']} waves)"},
         "errors": sum(x["status"] < 0 for x in res),
         "found": bool(r["found"]), "orth": r["orth"], "diag": r["diag"],
         "rounds": r["rounds"], "expanded": r["expanded"], "pops": r["pops"], "inserted": r["inserted"],
@@ -558,6 +558,8 @@ def main():
                     help="heaps (waves) of the plain whole-GPU solve of the plain workloads (0 = default)")
     ap.add_argument("--grid-heaps", action="store_true",
                     help="grids keep one sorted heap per wave instead of the team's bucket ring (comparison)")
+    ap.add_argument("--team-waves", type=int, default=0,
+                    help="deep-grid: waves of the grid's team (multiple of 16; 0 = the default one block)")
     ap.add_argument("--xcd-shared", action="store_true",
                     help="serving: every team class on every XCD (no XCDs apart for one-block grid teams)")
     ap.add_argument("--atomic-probe", action="store_true",
@@ -571,7 +573,12 @@ def main():
     if args.atomic_probe:
         import uastar_amd as U
         with U.Context(0) as ctx:
-            print(json.dumps({f"blocks_{b}_gatomics_per_s": ctx.atomic_rate(26, b, 64) for b in (256, 1024, 4096)}))
+            out = {f"blocks_{b}_gatomics_per_s": ctx.atomic_rate(26, b, 64) for b in (256, 1024, 4096)}
+            kinds = ("load_wg", "atomicmin_wg", "atomicmin_agent", "load_agent")
+            for lg, size in ((14, "128KiB"), (17, "1MiB"), (25, "256MiB")):
+                for k, name in enumerate(kinds):
+                    out[f"latency_ns_{name}_{size}"] = ctx.access_latency(lg, k)
+            print(json.dumps(out))
         return
     if args.gpus != 1:
         print("bench.py runs on one GPU: every query is solved on a single MI355X "

@@ -235,6 +235,10 @@ int ua_set_tuning(ua_ctx *ctx, const ua_tuning *t);
  * 32-bit atomicMins (agent scope) at hashed positions of a fresh array of 2^log2_words
  * words; *gops = atomics per nanosecond. The roofline of the search's relaxations. */
 int ua_debug_atomic_rate(ua_ctx *ctx, int log2_words, int blocks, int iters, double *gops);
+/* Measures the latency of one dependent access: one wave follows a random cycle over
+ * 2^log2_words 64-bit words, iters steps of kind 0 (workgroup-scope load), 1 (workgroup-scope
+ * atomicMin), 2 (agent-scope atomicMin) or 3 (agent-scope load); *ns = ns per step. */
+int ua_debug_access_latency(ua_ctx *ctx, int log2_words, int kind, int iters, double *ns);
 /* Runs a random insert/pop sequence on one wave's heap; see tests. */
 int ua_debug_heap_selftest(ua_ctx *ctx, const int32_t *ops, int nops, const uint64_t *keys, int64_t nkeys,
                            int cap_nodes, uint64_t *out_pops, int64_t out_cap, int32_t *final_size);

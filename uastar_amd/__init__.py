@@ -121,6 +121,7 @@ def lib():
                                           C.POINTER(C.c_double)]),
             "ua_launch_graph_error": (C.c_char_p, [vp]),
             "ua_debug_atomic_rate": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]),
+            "ua_debug_access_latency": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]),
             "ua_debug_heap_selftest": (C.c_int, [vp, vp, C.c_int, vp, i64, C.c_int, vp, i64,
                                                  C.POINTER(i32)]),
             "ua_autotune": (C.c_int, [vp, C.POINTER(Query), i64, C.POINTER(Tuning)]),
@@ -408,6 +409,12 @@ class Context:
         g = C.c_double()
         self._check(lib().ua_debug_atomic_rate(self.h, log2_words, blocks, iters, C.byref(g)))
         return g.value
+
+    def access_latency(self, log2_words, kind, iters=2000) -> float:
+        """ns per dependent access (kind 0 wg load, 1 wg atomicMin, 2 agent atomicMin, 3 agent load)."""
+        ns = C.c_double()
+        self._check(lib().ua_debug_access_latency(self.h, log2_words, kind, iters, C.byref(ns)))
+        return ns.value
 
     def heap_selftest(self, ops, keys, cap_nodes):
         ops = np.ascontiguousarray(ops, dtype=np.int32)

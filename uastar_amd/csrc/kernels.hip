@@ -285,8 +285,39 @@ __global__ __launch_bounds__(256) void k_atomic_probe(uint32_t *a, uint64_t mask
     if (acc == 0x12345678u) sink[0] = acc;
 }
 
+// Latency probe: one wave follows a random cycle of indices (a[i] holds the next index),
+// one dependent access a step: KIND 0 a workgroup-scope load, 1 a workgroup-scope
+// atomicMin that changes nothing, 2 the same at agent scope, 3 an agent-scope load.
+// out[0] = s_memrealtime ticks (100 MHz) of `iters` steps, out[1] the last index.
+template <int KIND>
+__global__ __launch_bounds__(64) void k_latency_probe(uint64_t *a, uint64_t mask, int iters, uint64_t *out) {
+    uint64_t idx = ((uint64_t)threadIdx.x * 977u) & mask;
+    const int64_t t0 = (int64_t)__builtin_amdgcn_s_memrealtime();
+    for (int i = 0; i < iters; ++i) {
+        uint64_t v;
+        if (KIND == 0) v = __hip_atomic_load(a + idx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        else if (KIND == 1) v = __hip_atomic_fetch_min(a + idx, ~0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        else if (KIND == 2) v = __hip_atomic_fetch_min(a + idx, ~0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        else v = __hip_atomic_load(a + idx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        idx = v & mask;
+    }
+    const int64_t t1 = (int64_t)__builtin_amdgcn_s_memrealtime();
+    if (threadIdx.x == 0) {
+        out[0] = (uint64_t)(t1 - t0);
+        out[1] = idx;
+    }
+}
+
 // ------------------------------------------------------------------ launchers
 static int grid_blocks(size_t n, int t) { return (int)((n + t - 1) / t); }
+
+hipError_t launch_latency_probe(uint64_t *a, uint64_t mask, int kind, int iters, uint64_t *out, hipStream_t st) {
+    if (kind == 0) hipLaunchKernelGGL(k_latency_probe<0>, dim3(1), dim3(64), 0, st, a, mask, iters, out);
+    else if (kind == 1) hipLaunchKernelGGL(k_latency_probe<1>, dim3(1), dim3(64), 0, st, a, mask, iters, out);
+    else if (kind == 2) hipLaunchKernelGGL(k_latency_probe<2>, dim3(1), dim3(64), 0, st, a, mask, iters, out);
+    else hipLaunchKernelGGL(k_latency_probe<3>, dim3(1), dim3(64), 0, st, a, mask, iters, out);
+    return hipGetLastError();
+}
 
 hipError_t launch_atomic_probe(uint32_t *a, uint64_t mask, int blocks, int iters, uint32_t *sink, hipStream_t st) {
     hipLaunchKernelGGL(k_atomic_probe, dim3(blocks), dim3(256), 0, st, a, mask, iters, sink);

@@ -24,6 +24,7 @@ hipError_t launch_teams(char *io, int blocks, uint64_t *segs, int mode, int set,
 // Path walks of finished queries, one lane each (see PathJob).
 hipError_t launch_paths(PathJob *jobs, int n, int32_t *out, hipStream_t st);
 hipError_t build_pdb(int n, uint8_t *pdb, unsigned long long *d_added, hipStream_t st);
+hipError_t launch_latency_probe(uint64_t *a, uint64_t mask, int kind, int iters, uint64_t *out, hipStream_t st);
 hipError_t launch_atomic_probe(uint32_t *a, uint64_t mask, int blocks, int iters, uint32_t *sink, hipStream_t st);
 hipError_t launch_heap_selftest(const int *ops, int nops, const uint64_t *keys, uint64_t *storage,
                                 int cap, uint64_t *out, int *status, hipStream_t st);

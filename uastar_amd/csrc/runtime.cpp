@@ -1145,3 +1145,35 @@ extern "C" int ua_debug_atomic_rate(ua_ctx *c, int log2_words, int blocks, int i
     *gops = (double)blocks * 256.0 * iters / ((double)ms * 1e6);
     return UA_OK;
 }
+
+// Dependent-access latency (k_latency_probe): one wave follows a random cycle over
+// 2^log2_words 64-bit words (built on the host, every entry the next index), `iters`
+// steps of access `kind` (0 workgroup load, 1 workgroup atomicMin, 2 agent atomicMin,
+// 3 agent load); *ns = nanoseconds per step of the second of two runs.
+extern "C" int ua_debug_access_latency(ua_ctx *c, int log2_words, int kind, int iters, double *ns) {
+    if (!c || !ns || log2_words < 6 || log2_words > 28 || kind < 0 || kind > 3 || iters < 1)
+        return fail(c, UA_ERR_INVALID, "bad arguments");
+    const uint64_t n = 1ull << log2_words;
+    std::vector<uint64_t> h(n);
+    uint64_t x = 88172645463325252ull;  // xorshift64 (Marsaglia): an in-range index per entry
+    for (uint64_t i = 0; i < n; ++i) {
+        x ^= x << 13;
+        x ^= x >> 7;
+        x ^= x << 17;
+        h[i] = x & (n - 1);
+    }
+    DevBuf a, out;
+    HIPCHK(c, a.ensure(n * 8));
+    HIPCHK(c, out.ensure(64));
+    HIPCHK(c, hipMemcpy(a.p, h.data(), n * 8, hipMemcpyHostToDevice));
+    uint64_t r[2] = {0, 0};
+    for (int rep = 0; rep < 2; ++rep) {
+        HIPCHK(c, launch_latency_probe(a.as<uint64_t>(), n - 1, kind, iters, out.as<uint64_t>(), c->st));
+        HIPCHK(c, hipMemcpyAsync(r, out.p, 16, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(c, hipStreamSynchronize(c->st));
+    }
+    a.release();
+    out.release();
+    *ns = (double)r[0] * 10.0 / iters;
+    return UA_OK;
+}
