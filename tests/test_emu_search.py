@@ -5,6 +5,7 @@ This checks the batch path's logic (one-block groups of k_teams, the group
 barrier and query hand-off, per-wave heaps with HBM levels) without a GPU.
 """
 import ctypes as C
+import json
 import os
 import subprocess
 
@@ -362,6 +363,47 @@ def test_emulated_bounded_weighted_puzzles(emu, oracle):
     for r in res[3:5]:  # bounded: pruned, no error
         assert r.status in (0, 1) and r.flags & 16
     assert res[5].status == -3  # unbounded: capacity error
+
+
+@pytest.mark.slow
+def test_emulated_fifteen_puzzles(emu, oracle):
+    """15-puzzles, whose heuristic sums two PDB partitions: a child's h is its parent's
+    with the moved tile's partition term swapped, the parent's second term read from its
+    item key. The four fixture boards with the fewest expansions give their stored move
+    counts, also when suspended every few rounds; a weighted run (wshift 1) stops at a
+    solution no shorter than the optimum and of its parity."""
+    pdb4 = oracle.pdb_full(4, 1, cache_dir="_cache")
+    with open(os.path.join(HERE, "golden", "c3_puzzles.json")) as f:
+        rows = sorted(json.load(f), key=lambda r: r["expanded"])[:4]
+    b = Batch(waves=4)
+    for r in rows:
+        b.add_puzzle(4, np.array(r["board"], np.uint8), pdb4, hash_log2=18)
+    b.add_puzzle(4, np.array(rows[0]["board"], np.uint8), pdb4, hash_log2=18)
+    b.descs[-1].wshift = 1
+    res = b.run(emu)
+    for r, row in zip(res[:4], rows):
+        assert r.status == 0 and r.found and (r.goal >> 32) == row["steps"], (row["seed"], r.status)
+    steps = res[4].goal >> 32
+    assert res[4].status == 0 and steps >= rows[0]["steps"] and (steps - rows[0]["steps"]) % 2 == 0
+    # the same boards suspended every 5 rounds: the resumed keys carry their terms
+    b2 = Batch(waves=4)
+    for r in rows[:2]:
+        b2.add_puzzle(4, np.array(r["board"], np.uint8), pdb4, hash_log2=18)
+    recs = [np.zeros(RESUME_WORDS_4, np.uint64) for _ in b2.descs]
+    for d, rec in zip(b2.descs, recs):
+        d.resume, d.round_budget = rec.ctypes.data, 5
+    final = [None] * len(b2.descs)
+    for _ in range(400):
+        idx = [i for i, f in enumerate(final) if f is None]
+        if not idx:
+            break
+        sub = Batch(waves=4)
+        sub.descs = [b2.descs[i] for i in idx]
+        for i, r in zip(idx, sub.run(emu)):
+            if r.status != 2:
+                final[i] = r
+    for r, row in zip(final, rows[:2]):
+        assert r is not None and r.status == 0 and (r.goal >> 32) == row["steps"], row["seed"]
 
 
 def test_emulated_heap_selftest(emu):

@@ -27,6 +27,7 @@ constexpr uint64_t KEY_MASK36 = (1ull << 36) - 1;
 constexpr double COST_LIMIT = 180000.0;
 constexpr uint64_t KEY_LIMIT = 180000ull << 36;   // every f key must stay below this
 constexpr uint64_t SENT = ~0ull;                  // empty heap slot / "infinity"
+constexpr int kPuzzleSlotBits = 40;               // hash slot field of a puzzle item key
 
 // Move table of uAstar (utils.hpp:120-121): 0..3 orthogonal, 4..7 diagonal.
 // The same order serves the puzzle's blank moves (indices 0..3).

@@ -158,6 +158,9 @@ void pack_board5(const uint8_t *t, uint64_t *w0, uint64_t *w1) {
 }
 
 int puzzle_fill_desc(ua_ctx *c, int n, const uint8_t *tiles, int heaps, int hash_log2, QueryDesc &q, Plan &p) {
+    if (hash_log2 < 1 || hash_log2 > kPuzzleSlotBits)
+        return fail(c, UA_ERR_INVALID, "puzzle hash of 2^%d slots: the item key holds %d slot bits", hash_log2,
+                    kPuzzleSlotBits);
     std::memset(&q, 0, sizeof q);
     q.kind = KIND_PUZZLE;
     q.n = n;

@@ -518,9 +518,18 @@ __device__ __forceinline__ int puzzle_h(const PTab &tb, uint64_t s, const uint8_
     return h;
 }
 
-// Item key: (f << 56) | ((255 - g) << 48) | slot  (ties: larger g first).
+// Item key: (f << 56) | ((255 - g) << 48) | (h1 << 40) | slot (ties: larger g first), h1
+// the state's second-partition PDB term (0 for N = 3). With the state's h = f - g (f - g >>
+// wshift when weighted) known from the key, a child's h needs one gather: a move shifts one
+// tile, so only that tile's partition term changes.
 // Hash: keys[slot] = board (0 = empty), vals[slot] = (g << 4) | dir (15 = start).
 // The side n (3 or 4) is a run-time value.
+constexpr uint32_t tile_mask(const int (&t)[8], int k) {
+    uint32_t m = 0;
+    for (int j = 0; j < k; ++j) m |= 1u << t[j];
+    return m;
+}
+constexpr uint32_t kPart1Mask4 = tile_mask(PSpec<4>::T[1], PSpec<4>::K[1]);  // tiles of the 15-puzzle's second partition
 struct PuzzleProb {
     static constexpr int MAXS = 4;
     static constexpr bool kContinue = false;
@@ -531,10 +540,12 @@ struct PuzzleProb {
     int n;
     uint64_t hmask;
     uint64_t goal_b;
+    uint32_t p1mask;  // tiles of the second partition (none for N = 3)
     __device__ __forceinline__ PuzzleProb(const QueryDesc &qd, const uint8_t *p) : q(qd), pdb(p), tb(kPTab[qd.n == 3 ? 0 : 1]) {
         n = qd.n == 3 ? 3 : 4;
         hmask = (1ull << q.hash_log2) - 1;
         goal_b = goal_board(n);
+        p1mask = n == 4 ? kPart1Mask4 : 0u;
     }
     __device__ bool trivial() const { return q.start_state == goal_b; }
     template <class SC>
@@ -558,8 +569,9 @@ struct PuzzleProb {
         uint32_t err = 0, atom = 0;
         uint64_t slot = find_or_insert(sc, q.start_state, err, atom);
         sc.store(&q.vals[slot], (uint32_t)15);
-        uint64_t h = (uint64_t)puzzle_h(tb, q.start_state, pdb);
-        return (h << 56) | (255ull << 48) | slot;
+        const uint64_t h = (uint64_t)puzzle_h(tb, q.start_state, pdb);
+        const uint64_t h1 = tb.np > 1 ? gptr(pdb)[tb.off[1] + pattern_rank_rt(tb, q.start_state, 1)] : 0u;
+        return (h << 56) | (255ull << 48) | (h1 << 40) | slot;
     }
     // a weighted search (memory-bounded scheme) stops at its first goal
     __device__ bool goal_done(uint64_t goal, uint64_t lb) const {
@@ -573,12 +585,14 @@ struct PuzzleProb {
     template <class SC>
     __device__ __forceinline__ void expand(const SC &sc, uint64_t it, uint64_t (&s)[MAXS], int &cnt,
                                            uint64_t &goalmin, uint32_t &err, Ctr &k) const {
-        uint64_t slot = it & ((1ull << 48) - 1);
+        uint64_t slot = it & ((1ull << kPuzzleSlotBits) - 1);
         if (slot > hmask) {  // never produced by a correct round: fail the query, touch nothing
             err |= FLAG_ABORT;
             return;
         }
         uint32_t gi = 255u - (uint32_t)((it >> 48) & 255u);
+        const int hp = (int)(it >> 56) - (int)(gi >> q.wshift);  // the state's h and its second term
+        const int h1p = (int)((it >> 40) & 255u);
         // the popped state's board and value may have been written from another block:
         // scoped loads (the stale test must see the value that pushed the key, or a lower one)
         uint64_t b = sc.load(gptr(q.best) + slot);
@@ -594,7 +608,7 @@ struct PuzzleProb {
         // flight together, and so are the atomicMins on their value words; the hash probes
         // (a load, then a CAS on an empty slot) stay one child at a time.
         uint64_t ns[4];
-        int h[4];
+        int h[4], h1[4];
         uint32_t legal = 0;
 #pragma unroll
         for (int d = 0; d < 4; ++d) {
@@ -607,7 +621,13 @@ struct PuzzleProb {
             const int nb = nx * n + ny;
             const uint64_t t = (b >> (4 * nb)) & 15ull;
             ns[d] = b + (t << (4 * bl)) - (t << (4 * nb));
-            h[d] = puzzle_h(tb, ns[d], pdb);
+            // the moved tile's partition term is the only one to change: one gather (both
+            // ranks are computed, so that the radix tables stay wave-uniform)
+            const bool second = (p1mask >> t) & 1u;
+            const uint32_t r0 = pattern_rank_rt(tb, ns[d], 0), r1 = pattern_rank_rt(tb, ns[d], 1);
+            const int term = gptr(pdb)[second ? tb.off[1] + r1 : tb.off[0] + r0];
+            h[d] = second ? hp - h1p + term : term + h1p;
+            h1[d] = second ? term : h1p;
         }
         uint64_t ts[4];
 #pragma unroll
@@ -640,7 +660,7 @@ struct PuzzleProb {
                 goalmin = goalmin < gk ? goalmin : gk;
             } else {
                 const uint32_t f = (ng >> q.wshift) + (uint32_t)h[d];
-                s[d] = ((uint64_t)f << 56) | ((uint64_t)(255u - ng) << 48) | ts[d];
+                s[d] = ((uint64_t)f << 56) | ((uint64_t)(255u - ng) << 48) | ((uint64_t)h1[d] << 40) | ts[d];
                 ++cnt;
             }
         }
@@ -768,7 +788,7 @@ struct PuzzleProb5 {
     template <class SC>
     __device__ __forceinline__ void expand(const SC &sc, uint64_t it, uint64_t (&s)[MAXS], int &cnt,
                                            uint64_t &goalmin, uint32_t &err, Ctr &k) const {
-        uint64_t slot = it & ((1ull << 40) - 1);
+        uint64_t slot = it & ((1ull << kPuzzleSlotBits) - 1);
         if (slot > hmask) {  // never produced by a correct round: fail the query, touch nothing
             err |= FLAG_ABORT;
             return;
