@@ -151,6 +151,13 @@ inline T __hip_atomic_fetch_min(T *p, U v, int, int) {
     return cur;
 }
 template <class T, class U>
+inline T __hip_atomic_fetch_max(T *p, U v, int, int) {
+    T cur = __atomic_load_n(p, __ATOMIC_SEQ_CST);
+    while ((T)v > cur && !__atomic_compare_exchange_n(p, &cur, (T)v, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST)) {
+    }
+    return cur;
+}
+template <class T, class U>
 inline bool __hip_atomic_compare_exchange_strong(T *p, T *expected, U desired, int, int, int) {
     return __atomic_compare_exchange_n(p, expected, (T)desired, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST);
 }

@@ -92,8 +92,7 @@ class Batch:
         self.descs.append(d)
 
     def add_puzzle(self, n, tiles, pdb, hash_log2=17, items=65536):
-        keys = np.zeros(1 << hash_log2, np.uint64)
-        vals = np.full(1 << hash_log2, 0xFFFFFFFF, np.uint32)
+        slots = np.zeros(2 << hash_log2, np.uint64)  # 16-byte slots: board, complemented value
         per_heap = (items + 64 * self.waves - 1) // (64 * self.waves)
         cap = max(per_heap + 1, 4)
         heap = np.zeros(self.waves * (cap - 1) * 64, np.uint64)
@@ -101,9 +100,9 @@ class Batch:
         for i, t in enumerate(tiles):
             s |= int(t) << (4 * i)
         d = QueryDesc(kind=2, n=n, hash_log2=hash_log2, heap_cap=cap, max_rounds=self.max_rounds, heaps=self.waves)
-        d.best, d.vals, d.heap, d.start_state = keys.ctypes.data, vals.ctypes.data, heap.ctypes.data, s
+        d.best, d.heap, d.start_state = slots.ctypes.data, heap.ctypes.data, s
         d.pdb = pdb.ctypes.data
-        self.keep += [keys, vals, heap, pdb]
+        self.keep += [slots, heap, pdb]
         self.descs.append(d)
 
     def run(self, lib):

@@ -48,16 +48,15 @@ __device__ int64_t walk_grid(const uint64_t *best, int H, int W, int sx, int sy,
 }
 
 // Blank-move indices goal..start (puzzle/GPU-kernel.cuh:462-509 semantics), for 3x3 / 4x4
-// boards of one u64.
+// boards of one u64 in 16-byte hash slots (board, complemented value).
 template <int N>
-__device__ int64_t walk_moves(const uint64_t *keys, const uint32_t *vals, int hash_log2, uint64_t goal_slot,
-                              int32_t *dirs, int64_t cap) {
+__device__ int64_t walk_moves(const uint64_t *slots, int hash_log2, uint64_t goal_slot, int32_t *dirs, int64_t cap) {
     uint64_t hmask = (1ull << hash_log2) - 1;
     uint64_t slot = goal_slot;
-    uint64_t b = keys[slot];
+    uint64_t b = slots[2 * slot];
     int64_t c = 0;
     while (c <= cap) {
-        int d = (int)(vals[slot] & 15u);
+        int d = (int)(~(uint32_t)slots[2 * slot + 1] & 15u);
         if (d == 15) return c;
         if (d > 3 || c == cap) break;
         dirs[c++] = d;
@@ -69,7 +68,7 @@ __device__ int64_t walk_moves(const uint64_t *keys, const uint32_t *vals, int ha
         b = b + (t << (4 * bl)) - (t << (4 * pb));
         slot = mix64(b) & hmask;
         int i = 0;
-        for (; i < PuzzleProb::MAXPROBE && keys[slot] != b; ++i) slot = (slot + 1) & hmask;
+        for (; i < PuzzleProb::MAXPROBE && slots[2 * slot] != b; ++i) slot = (slot + 1) & hmask;
         if (i == PuzzleProb::MAXPROBE) break;
     }
     return -1;
@@ -112,8 +111,8 @@ __global__ void k_paths(PathJob *jobs, int n, int32_t *out) {
     int32_t *o = out + j.off;
     if (j.kind == KIND_GRID) j.got = walk_grid(j.best, j.H, j.W, j.sx, j.sy, j.ex, j.ey, o, j.len);
     else if (j.n == 5) j.got = walk_moves5(j.best, j.vals, j.hash_log2, j.goal_slot, o, j.len);
-    else if (j.n == 4) j.got = walk_moves<4>(j.best, j.vals, j.hash_log2, j.goal_slot, o, j.len);
-    else j.got = walk_moves<3>(j.best, j.vals, j.hash_log2, j.goal_slot, o, j.len);
+    else if (j.n == 4) j.got = walk_moves<4>(j.best, j.hash_log2, j.goal_slot, o, j.len);
+    else j.got = walk_moves<3>(j.best, j.hash_log2, j.goal_slot, o, j.len);
 }
 
 // ------------------------------------------------------------------ grid generator

@@ -175,8 +175,10 @@ int puzzle_fill_desc(ua_ctx *c, int n, const uint8_t *tiles, int heaps, int hash
     int64_t per_heap = (items + 64ll * heaps - 1) / (64ll * heaps);
     q.heap_cap = (int)std::max<int64_t>(per_heap + 1, 4);
     p = Plan{};
-    p.best_bytes = (size_t)(n == 5 ? 16 : 8) << hash_log2;  // keys: one word a board, two for n = 5
-    p.vals_bytes = (size_t)4 << hash_log2;
+    // n = 5: two key words a slot and a value array; n = 3, 4: 16-byte slots of board and
+    // complemented value, empty when zeroed
+    p.best_bytes = (size_t)16 << hash_log2;
+    p.vals_bytes = n == 5 ? (size_t)4 << hash_log2 : 0;
     p.heap_bytes = (size_t)heaps * (size_t)(q.heap_cap - 1) * 512;
     return UA_OK;
 }
@@ -270,7 +272,7 @@ static int run_wide(ua_ctx *c, QueryDesc &q, const Plan &p) {
         HIPCHK(c, launch_grid_mark_blocked(q.best, q.bitmap, q.H, q.W, q.wpr, c->st));
     } else {
         HIPCHK(c, hipMemsetAsync(q.best, 0, p.best_bytes, c->st));
-        HIPCHK(c, hipMemsetAsync(q.vals, 0xFF, p.vals_bytes, c->st));
+        if (p.vals_bytes) HIPCHK(c, hipMemsetAsync(q.vals, 0xFF, p.vals_bytes, c->st));
     }
     LaunchShape shape{};
     shape.groups[0] = 1;
@@ -1017,7 +1019,7 @@ extern "C" int ua_batch_solve(ua_ctx *c, const ua_query *qs, int64_t nq, ua_resu
             held += arena.n;
             int arena_idx = (int)c->b_arenas.size() - 1;
             char *base = arena.as<char>();
-            // layout: [grid best tables | puzzle vals] filled with 0xFF, then [puzzle keys] = 0, then heaps
+            // layout: [grid best tables | 5x5 puzzle vals] filled with 0xFF, then [puzzle slots] = 0, then heaps
             std::vector<QueryDesc> sub;
             size_t off = 0;
             for (size_t k = pos; k < end; ++k) {

@@ -80,6 +80,11 @@ struct Scope {
                   : __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     template <class T>
+    __device__ __forceinline__ T fmax(T *p, T v) const {
+        return wg ? __hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
+                  : __hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    template <class T>
     __device__ __forceinline__ T fadd(T *p, T v) const {
         return wg ? __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
                   : __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -116,6 +121,11 @@ struct Scope {
     __device__ __forceinline__ T fmin(UA_GLOBAL T *p, T v) const {
         return wg ? __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
                   : __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    template <class T>
+    __device__ __forceinline__ T fmax(UA_GLOBAL T *p, T v) const {
+        return wg ? __hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
+                  : __hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     __device__ __forceinline__ bool cas(UA_GLOBAL uint64_t *p, uint64_t &expect, uint64_t v) const {
         return wg ? __hip_atomic_compare_exchange_strong(p, &expect, v, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
@@ -522,7 +532,10 @@ __device__ __forceinline__ int puzzle_h(const PTab &tb, uint64_t s, const uint8_
 // the state's second-partition PDB term (0 for N = 3). With the state's h = f - g (f - g >>
 // wshift when weighted) known from the key, a child's h needs one gather: a move shifts one
 // tile, so only that tile's partition term changes.
-// Hash: keys[slot] = board (0 = empty), vals[slot] = (g << 4) | dir (15 = start).
+// Hash: slot s is 16 bytes, best[2 s] = board (0 = empty) and the low half of best[2 s + 1]
+// = ~((g << 4) | dir) (dir 15 = start). The value is kept complemented so that a zeroed
+// table is empty (0 = no value) and lowering it is an atomicMax; the expansion reads a
+// popped state's board and value from one line.
 // The side n (3 or 4) is a run-time value.
 constexpr uint32_t tile_mask(const int (&t)[8], int k) {
     uint32_t m = 0;
@@ -548,17 +561,21 @@ struct PuzzleProb {
         p1mask = n == 4 ? kPart1Mask4 : 0u;
     }
     __device__ bool trivial() const { return q.start_state == goal_b; }
+    __device__ __forceinline__ UA_GLOBAL uint64_t *board_word(uint64_t slot) const { return gptr(q.best) + 2 * slot; }
+    __device__ __forceinline__ UA_GLOBAL uint32_t *value_word(uint64_t slot) const {
+        return reinterpret_cast<UA_GLOBAL uint32_t *>(gptr(q.best) + (2 * slot + 1));
+    }
     template <class SC>
     __device__ __forceinline__ uint64_t find_or_insert(const SC &sc, uint64_t b, uint32_t &err,
                                                        uint32_t &atom) const {
         uint64_t slot = mix64(b) & hmask;
         for (int i = 0; i < MAXPROBE; ++i) {
-            uint64_t k = sc.load_weak(gptr(q.best) + slot);
+            uint64_t k = sc.load_weak(board_word(slot));
             if (k == b) return slot;
             if (k == 0) {
                 uint64_t exp = 0;
                 ++atom;
-                if (sc.cas(gptr(q.best) + slot, exp, b) || exp == b) return slot;
+                if (sc.cas(board_word(slot), exp, b) || exp == b) return slot;
             }
             slot = (slot + 1) & hmask;
         }
@@ -568,7 +585,7 @@ struct PuzzleProb {
     __device__ uint64_t seed(const Scope &sc) const {
         uint32_t err = 0, atom = 0;
         uint64_t slot = find_or_insert(sc, q.start_state, err, atom);
-        sc.store(&q.vals[slot], (uint32_t)15);
+        sc.store(value_word(slot), ~(uint32_t)15);
         const uint64_t h = (uint64_t)puzzle_h(tb, q.start_state, pdb);
         const uint64_t h1 = tb.np > 1 ? gptr(pdb)[tb.off[1] + pattern_rank_rt(tb, q.start_state, 1)] : 0u;
         return (h << 56) | (255ull << 48) | (h1 << 40) | slot;
@@ -595,8 +612,8 @@ struct PuzzleProb {
         const int h1p = (int)((it >> 40) & 255u);
         // the popped state's board and value may have been written from another block:
         // scoped loads (the stale test must see the value that pushed the key, or a lower one)
-        uint64_t b = sc.load(gptr(q.best) + slot);
-        uint32_t v = sc.load(gptr(q.vals) + slot);
+        uint64_t b = sc.load(board_word(slot));
+        uint32_t v = ~sc.load(value_word(slot));
         uint32_t gb = v >> 4;
         if (gb != gi) return;  // stale
         ++k.exp;
@@ -648,7 +665,7 @@ struct PuzzleProb {
             old[d] = 0;  // no atomic: never an improvement
             if (ts[d] != SENT) {
                 ++k.atom;
-                old[d] = sc.fmin(gptr(q.vals) + ts[d], (ng << 4) | (uint32_t)d);
+                old[d] = ~sc.fmax(value_word(ts[d]), ~((ng << 4) | (uint32_t)d));
             }
         }
 #pragma unroll
@@ -1110,16 +1127,57 @@ struct BucketMeta {
 };
 static_assert(sizeof(BucketMeta) <= kBucketMetaWords * 8, "bucket counters fit their reserved words");
 
+// Relaxed atomics at workgroup scope fixed at compile time, for the best words of a
+// one-block team's grid (the team is their only user).
+struct WgScope {
+    template <class T>
+    __device__ __forceinline__ T load(const UA_GLOBAL T *p) const {
+        return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    template <class T>
+    __device__ __forceinline__ T load_weak(const UA_GLOBAL T *p) const {
+        return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    template <class T>
+    __device__ __forceinline__ void store(UA_GLOBAL T *p, T v) const {
+        __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    template <class T>
+    __device__ __forceinline__ T fmin(UA_GLOBAL T *p, T v) const {
+        return __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+};
+// LDS words of the ring's counters and round slots: ds_* instructions with 32-bit
+// addresses, not flat ones (a flat access takes the vector memory path even when it
+// resolves to LDS).
+template <class T>
+__device__ __forceinline__ T lds_ld(const UA_LDS T *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+template <class T>
+__device__ __forceinline__ void lds_st(UA_LDS T *p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+template <class T>
+__device__ __forceinline__ T lds_add(UA_LDS T *p, T v) {
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// The ring runs on one-block teams only (the planner gives a grid a ring only for a
+// 16-wave team): counters and round slots in LDS, block barriers. The counters take the
+// start of the block's LDS insert-buffer area (that path does not use the buffer), the
+// continuation's parked successors follow them.
+constexpr int kLdsSegCap = kBlockWaves * 64 * 8;  // keys of one round's LDS insert-buffer half: 16 waves x 64 pops x 8 moves
+static_assert(kBucketMetaWords + GridProb::MAXS * kBlockWaves * 64 <= 2 * kLdsSegCap,
+              "ring counters and parked successors fit the LDS buffer");
 template <class Team>
 __device__ __forceinline__ void team_solve_buckets(const QueryDesc &q, QueryResult *res, const Team &team,
-                                                   const GridProb &P, BucketMeta *lds_meta) {
-    const Scope sc = team.sc;
+                                                   const GridProb &P, UA_LDS BucketMeta *bm,
+                                                   UA_LDS RoundSlot *slots) {
+    const WgScope sc;
     const int l = lane_id();
     const int tw = team.tw, W = team.W;
     const int tid = (int)threadIdx.x, nthr = (int)blockDim.x;
-    const bool one_block = team.bpg == 1;
-    const bool meta_lds = one_block && lds_meta != nullptr;
-    RoundSlot *const slots = team.m.slot;
     uint64_t *const stats = team.m.stats;
     if (P.trivial()) {  // start == goal: cost 0, no search
         if (team.leader()) {
@@ -1136,47 +1194,46 @@ __device__ __forceinline__ void team_solve_buckets(const QueryDesc &q, QueryResu
     const int bsh = q.cbits + q.bshift;  // key >> bsh = the key's bucket
     const int blog = q.bkt_log2;
     const uint32_t bmask = (1u << blog) - 1, bcap = 1u << blog;
-    BucketMeta *const bm = meta_lds ? lds_meta : reinterpret_cast<BucketMeta *>(q.heap);
     UA_GLOBAL uint64_t *const store = gptr(q.heap) + kBucketMetaWords;
+    // per lane, MAXS successors parked in LDS after the counters while children expand
+    UA_LDS uint64_t *const stash = reinterpret_cast<UA_LDS uint64_t *>(bm) + kBucketMetaWords;
     const bool resumed = q.resume != nullptr && q.resume[0] != 0;
     const int r0 = resumed ? (int)q.resume[1] : 0;
-    // resume record after its header: the lowest bucket, then (counters in LDS) tail[] and
-    // the next round's consumed counts as 32-bit words
+    // resume record after its header: the lowest bucket, then tail[] and the next round's
+    // consumed counts as 32-bit words
     uint32_t *const rec32 = q.resume ? reinterpret_cast<uint32_t *>(q.resume + RESUME_HEADER + 1) : nullptr;
     uint64_t cur;  // the lowest bucket that can hold a key (wave-uniform)
     if (resumed) {
         cur = q.resume[RESUME_HEADER];
-        if (meta_lds)
-            for (int i = tid; i < kBuckets; i += nthr) {
-                bm->tail[i] = rec32[i];
-                bm->head[r0 & 1][i] = rec32[kBuckets + i];
-            }
+        for (int i = tid; i < kBuckets; i += nthr) {
+            lds_st(&bm->tail[i], rec32[i]);
+            lds_st(&bm->head[r0 & 1][i], rec32[kBuckets + i]);
+        }
     } else {
         cur = P.item(octile_key(q.sx, q.sy, q.ex, q.ey), q.sx, q.sy) >> bsh;
-        if (team.gblock == 0)
-            for (int i = tid; i < kBuckets; i += nthr) {
-                sc.store(&bm->tail[i], 0u);
-                sc.store(&bm->head[0][i], 0u);
-            }
+        for (int i = tid; i < kBuckets; i += nthr) {
+            lds_st(&bm->tail[i], 0u);
+            lds_st(&bm->head[0][i], 0u);
+        }
     }
     if (team.leader()) {
         for (int i = 0; i < 2; ++i) {
-            sc.store(&slots[i].lb, SENT);
-            sc.store(&slots[i].goal, SENT);
-            sc.store(&slots[i].cnt, 0u);
-            sc.store(&slots[i].err, 0u);
+            lds_st(&slots[i].lb, SENT);
+            lds_st(&slots[i].goal, SENT);
+            lds_st(&slots[i].cnt, 0u);
+            lds_st(&slots[i].err, 0u);
         }
-        for (int i = 0; i < 4; ++i) sc.store(&stats[i], resumed ? q.resume[3 + i] : (uint64_t)0);
-        sc.store(&stats[4], resumed ? q.resume[8] : (uint64_t)0);
+        for (int i = 0; i < 4; ++i) team.sc.store(&stats[i], resumed ? q.resume[3 + i] : (uint64_t)0);
+        team.sc.store(&stats[4], resumed ? q.resume[8] : (uint64_t)0);
     }
-    if (!team.sync([] {})) return;
+    __syncthreads();
     if (!resumed && team.leader()) {  // the start's record and key
-        const uint64_t key0 = P.seed(sc);
+        const uint64_t key0 = P.seed(team.sc);
         const uint32_t ring = (uint32_t)cur & RM;
         sc.store(store + ((uint64_t)ring << blog), key0);
-        sc.store(&bm->tail[ring], 1u);
+        lds_st(&bm->tail[ring], 1u);
     }
-    if (!team.sync([] {})) return;
+    __syncthreads();
 
     uint32_t n_pop = 0;
     Ctr k;
@@ -1189,24 +1246,18 @@ __device__ __forceinline__ void team_solve_buckets(const QueryDesc &q, QueryResu
     TeamVars *const tvp = team.tv;
     int r = r0;
     for (;; ++r) {
-        RoundSlot *const cs = &slots[r & 1];
-        uint32_t *const hcur = bm->head[r & 1];
-        uint32_t *const hnext = bm->head[(r + 1) & 1];
+        UA_LDS RoundSlot *const cs = &slots[r & 1];
+        UA_LDS uint32_t *const hcur = bm->head[r & 1];
+        UA_LDS uint32_t *const hnext = bm->head[(r + 1) & 1];
         // (A) the ring's counters in window order: element i of lane l is bucket cur + 4 l + i
         uint32_t hd[4], lv[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const uint32_t ring = ((uint32_t)cur + 4u * (uint32_t)l + (uint32_t)i) & RM;
-            const uint32_t t = sc.load(&bm->tail[ring]);
-            hd[i] = sc.load(&hcur[ring]);
-            lv[i] = t - hd[i];
+            hd[i] = lds_ld(&hcur[ring]);
+            lv[i] = lds_ld(&bm->tail[ring]) - hd[i];
         }
-        if (one_block) {
-            __syncthreads();
-        } else if (!team.sync([] {})) {
-            status = ST_ABORT;
-            break;
-        }
+        __syncthreads();
         // (B) plan
         int min_d = kBuckets;
 #pragma unroll
@@ -1232,13 +1283,14 @@ __device__ __forceinline__ void team_solve_buckets(const QueryDesc &q, QueryResu
             break;
         }
         if (team.leader()) {  // the next round's slot, and this round's time budget
-            RoundSlot *const ns = &slots[(r + 1) & 1];
-            sc.store(&ns->lb, SENT);
-            sc.store(&ns->goal, SENT);
-            sc.store(&ns->cnt, 0u);
-            sc.store(&ns->err, 0u);
+            UA_LDS RoundSlot *const ns = &slots[(r + 1) & 1];
+            lds_st(&ns->lb, SENT);
+            lds_st(&ns->goal, SENT);
+            lds_st(&ns->cnt, 0u);
+            lds_st(&ns->err, 0u);
             const uint64_t dl = __hip_atomic_load(&tvp->deadline, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (dl && (uint64_t)now_ticks() > dl) sc.fold_or(&cs->err, (uint32_t)FLAG_TIMEUP);
+            if (dl && (uint64_t)now_ticks() > dl)
+                __hip_atomic_fetch_or(&cs->err, (uint32_t)FLAG_TIMEUP, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         }
         // the window's keys, clamped to pmax in window order
         uint32_t tk[4], lsum = 0;
@@ -1263,7 +1315,7 @@ __device__ __forceinline__ void team_solve_buckets(const QueryDesc &q, QueryResu
         if (tw == 0) {
 #pragma unroll
             for (int i = 0; i < 4; ++i)
-                sc.store(&hnext[((uint32_t)cur + 4u * (uint32_t)l + (uint32_t)i) & RM], hd[i] + tk[i]);
+                lds_st(&hnext[((uint32_t)cur + 4u * (uint32_t)l + (uint32_t)i) & RM], hd[i] + tk[i]);
         }
         // (C) the key of rank j = 64 w + l: the lane L that holds it (the largest L with
         // exc_L <= j), then its element
@@ -1301,7 +1353,6 @@ __device__ __forceinline__ void team_solve_buckets(const QueryDesc &q, QueryResu
             const uint32_t ring = ((uint32_t)cur + 4u * (uint32_t)L + (uint32_t)e) & RM;
             it = sc.load(store + (((uint64_t)ring << blog) | ((hsel + off) & bmask)));
         }
-        const uint32_t cur0 = (uint32_t)cur;  // the snapshot's first ring bucket
         cur = blo;
         // (D) expansion
         uint64_t s[GridProb::MAXS];
@@ -1317,12 +1368,13 @@ __device__ __forceinline__ void team_solve_buckets(const QueryDesc &q, QueryResu
         }
         if (it != SENT) P.expand(sc, it, s, c, goalmin, err, k, &gk, best_goal);
         // continuation only in a saturated round (the window held more keys than the round
-        // takes), so that the chains run ahead of a best-first frontier
-        uint64_t keep = SENT;
+        // takes), so that the chains run ahead of a best-first frontier. While the children
+        // are expanded, the other lanes' successors wait in LDS rather than in registers.
+        uint64_t keep = SENT;  // the child's sibling, if any
         int dc = 0;
         const uint64_t child = it != SENT && (uint32_t)tot > pmax ? P.pick_child(it, s, c, &dc) : SENT;
         if (__ballot(child != SENT)) {
-            if (child != SENT) {  // the child's sibling, if any, waits in `keep`
+            if (child != SENT) {
 #pragma unroll
                 for (int i = 0; i < GridProb::MAXS; ++i) {
                     if (i != dc && s[i] < keep) keep = s[i];
@@ -1330,6 +1382,13 @@ __device__ __forceinline__ void team_solve_buckets(const QueryDesc &q, QueryResu
                 }
                 c = keep != SENT;
                 P.expand_child(sc, child, gk + (dc < 4 ? KEY_ONE : KEY_SQRT2), s, c, goalmin, err, k);
+            } else {
+#pragma unroll
+                for (int i = 0; i < GridProb::MAXS; ++i) stash[i * nthr + tid] = s[i];
+            }
+            if (child == SENT) {
+#pragma unroll
+                for (int i = 0; i < GridProb::MAXS; ++i) s[i] = stash[i * nthr + tid];
             }
         }
         // (E) appends (s[] and the sibling kept by continuation): every key's tail add is
@@ -1347,27 +1406,15 @@ __device__ __forceinline__ void team_solve_buckets(const QueryDesc &q, QueryResu
                 key = SENT;
                 continue;
             }
-            pos[i] = sc.fadd(&bm->tail[(uint32_t)b & RM], 1u);
+            pos[i] = lds_add(&bm->tail[(uint32_t)b & RM], 1u);
         }
 #pragma unroll
         for (int i = 0; i < NK; ++i) {
             const uint64_t key = i < GridProb::MAXS ? s[i] : keep;
-            const uint32_t ring = (uint32_t)(key >> bsh) & RM;
-            // the ring bucket's consumed count at this round's start: from LDS, or from the
-            // lane of the snapshot that holds it (every lane runs the shuffles)
-            uint32_t h;
-            if (meta_lds) {
-                h = key != SENT ? bm->head[r & 1][ring] : 0u;
-            } else {
-                const uint32_t d = (ring - cur0) & RM;
-                const uint32_t a0 = (uint32_t)__shfl((int)hd[0], (int)(d >> 2)),
-                               a1 = (uint32_t)__shfl((int)hd[1], (int)(d >> 2)),
-                               a2 = (uint32_t)__shfl((int)hd[2], (int)(d >> 2)),
-                               a3 = (uint32_t)__shfl((int)hd[3], (int)(d >> 2));
-                h = (d & 3) == 0 ? a0 : (d & 3) == 1 ? a1 : (d & 3) == 2 ? a2 : a3;
-            }
             if (key == SENT) continue;
-            if (pos[i] - h >= bcap) {  // full: at least this many of its keys are unconsumed
+            const uint32_t ring = (uint32_t)(key >> bsh) & RM;
+            // at least this many of the bucket's keys are unconsumed: full
+            if (pos[i] - lds_ld(&hcur[ring]) >= bcap) {
                 err |= FLAG_CAPACITY;
                 continue;
             }
@@ -1375,32 +1422,16 @@ __device__ __forceinline__ void team_solve_buckets(const QueryDesc &q, QueryResu
         }
         if (__ballot(goalmin != SENT)) {
             goalmin = wave_min64(goalmin);
-            if (l == 0) sc.fmin(&cs->goal, goalmin);
+            if (l == 0) __hip_atomic_fetch_min(&cs->goal, goalmin, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         }
         if (__ballot(err != 0)) {
             for (int m = 32; m > 0; m >>= 1) err |= __shfl_xor(err, m);
-            if (l == 0) sc.fold_or(&cs->err, err);
+            if (l == 0) __hip_atomic_fetch_or(&cs->err, err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         }
-        uint64_t g;
-        uint32_t rerr;
-        if (one_block) {  // the slot is in LDS: every wave reads the same values after the barrier
-            __syncthreads();
-            g = cs->goal;
-            rerr = cs->err;
-        } else {  // one lane reads the slot for the block, so every wave decides on the same values
-            if (!team.sync([] {})) {
-                status = ST_ABORT;
-                break;
-            }
-            RoundSlot *const view = team.m.view;
-            if (tid == 0) {
-                view->goal = sc.load(&cs->goal);
-                view->err = sc.load(&cs->err);
-            }
-            __syncthreads();
-            g = view->goal;
-            rerr = view->err;
-        }
+        // every wave reads the same slot values after the barrier
+        __syncthreads();
+        const uint64_t g = lds_ld(&cs->goal);
+        const uint32_t rerr = lds_ld(&cs->err);
         best_goal = best_goal < g ? best_goal : g;
         pruned |= rerr & FLAG_PRUNED;
         if (rerr & ~(uint32_t)(FLAG_TIMEUP | FLAG_PRUNED)) {
@@ -1416,26 +1447,26 @@ __device__ __forceinline__ void team_solve_buckets(const QueryDesc &q, QueryResu
     const int64_t t_pop = wave_sum64(n_pop), t_exp = wave_sum64(k.exp), t_gen = wave_sum64(k.gen),
                   t_ins = wave_sum64(k.ins), t_atom = wave_sum64(k.atom);
     if (l == 0) {
-        sc.fadd(&stats[0], (uint64_t)t_pop);
-        sc.fadd(&stats[1], (uint64_t)t_exp);
-        sc.fadd(&stats[2], (uint64_t)t_gen);
-        sc.fadd(&stats[3], (uint64_t)t_ins);
-        sc.fadd(&stats[4], (uint64_t)t_atom);
+        team.sc.fadd(&stats[0], (uint64_t)t_pop);
+        team.sc.fadd(&stats[1], (uint64_t)t_exp);
+        team.sc.fadd(&stats[2], (uint64_t)t_gen);
+        team.sc.fadd(&stats[3], (uint64_t)t_ins);
+        team.sc.fadd(&stats[4], (uint64_t)t_atom);
     }
-    if (status == ST_SUSPENDED && meta_lds) {  // the ring after round r, with the next round's consumed counts
+    if (status == ST_SUSPENDED) {  // the ring after round r, with the next round's consumed counts
         for (int i = tid; i < kBuckets; i += nthr) {
-            rec32[i] = bm->tail[i];
-            rec32[kBuckets + i] = bm->head[(r + 1) & 1][i];
+            rec32[i] = lds_ld(&bm->tail[i]);
+            rec32[kBuckets + i] = lds_ld(&bm->head[(r + 1) & 1][i]);
         }
     }
-    team.sync([] {});
+    __syncthreads();
     if (team.leader()) {
         if (status == ST_SUSPENDED) {
             q.resume[1] = (uint64_t)(r + 1);
             q.resume[2] = best_goal;
-            for (int i = 0; i < 4; ++i) q.resume[3 + i] = sc.load(&stats[i]);
+            for (int i = 0; i < 4; ++i) q.resume[3 + i] = team.sc.load(&stats[i]);
             q.resume[7] = pruned;
-            q.resume[8] = sc.load(&stats[4]);
+            q.resume[8] = team.sc.load(&stats[4]);
             q.resume[RESUME_HEADER] = cur;
             q.resume[0] = 1;
         }
@@ -1443,11 +1474,11 @@ __device__ __forceinline__ void team_solve_buckets(const QueryDesc &q, QueryResu
         res->found = found;
         res->goal = best_goal;
         res->rounds = r + 1;
-        res->pops = (int64_t)sc.load(&stats[0]);
-        res->expanded = (int64_t)sc.load(&stats[1]);
-        res->generated = (int64_t)sc.load(&stats[2]);
-        res->inserted = (int64_t)sc.load(&stats[3]) + 1;
-        res->atomics = (int64_t)sc.load(&stats[4]);
+        res->pops = (int64_t)team.sc.load(&stats[0]);
+        res->expanded = (int64_t)team.sc.load(&stats[1]);
+        res->generated = (int64_t)team.sc.load(&stats[2]);
+        res->inserted = (int64_t)team.sc.load(&stats[3]) + 1;
+        res->atomics = (int64_t)team.sc.load(&stats[4]);
         res->flags = pruned;
     }
 }
@@ -1470,7 +1501,14 @@ __device__ __forceinline__ void dispatch(const QueryDesc &q, QueryResult *res, c
         team_solve(q, res, team, P);
     } else if (q.kind == KIND_GRID) {
         GridProb P(q);
-        team_solve_buckets(q, res, team, P, static_cast<BucketMeta *>(team.m.aux));
+        if (team.bpg != 1 || team.m.aux == nullptr) {  // a ring needs a one-block team with LDS counters
+            if (team.leader()) {
+                res->status = ST_INVALID;
+                res->found = 0;
+            }
+            return;
+        }
+        team_solve_buckets(q, res, team, P, (UA_LDS BucketMeta *)team.m.aux, (UA_LDS RoundSlot *)team.m.slot);
     } else {
         PuzzleProb P(q, q.pdb);
         team_solve(q, res, team, P);
@@ -1514,8 +1552,6 @@ __device__ __forceinline__ void group_loop(const QueryDesc *qs, QueryResult *rs,
     }
 }
 
-// Keys of one round's insert buffer half that fit in LDS: 16 waves x 64 pops x 8 moves.
-constexpr int kLdsSegCap = kBlockWaves * 64 * 8;
 
 // Body of k_teams for one block: find this block's class and group from the launch
 // shape, then run the group's query loop.

@@ -174,15 +174,15 @@ int plan_job(ua_ctx *c, ServeEngine *e, Job &j, bool *fits) {
     j.d.heap = reinterpret_cast<uint64_t *>(base + o);
     o += align256(j.plan.heap_bytes);
     j.d.resume = reinterpret_cast<uint64_t *>(base + o);
-    // fresh search state: grid best = all ones but blocked cells' open bit; puzzle keys = 0,
-    // vals = all ones; no resume record
+    // fresh search state: grid best = all ones but blocked cells' open bit; puzzle slots = 0
+    // (5x5: keys = 0, vals = all ones); no resume record
     hipError_t he;
     if (j.d.kind == KIND_GRID) {
         he = hipMemsetAsync(j.d.best, 0xFF, j.plan.best_bytes, c->st);
         if (he == hipSuccess) he = launch_grid_mark_blocked(j.d.best, j.d.bitmap, j.d.H, j.d.W, j.d.wpr, c->st);
     } else {
         he = hipMemsetAsync(j.d.best, 0, j.plan.best_bytes, c->st);
-        if (he == hipSuccess) he = hipMemsetAsync(j.d.vals, 0xFF, j.plan.vals_bytes, c->st);
+        if (he == hipSuccess && j.plan.vals_bytes) he = hipMemsetAsync(j.d.vals, 0xFF, j.plan.vals_bytes, c->st);
     }
     if (he == hipSuccess) he = hipMemsetAsync(j.d.resume, 0, 64, c->st);
     if (he != hipSuccess) {  // the job is not admitted: give its region back

@@ -24,8 +24,10 @@ namespace ua {
 // for an LDS result does not wait for HBM loads still in flight.
 #ifdef UA_EMU
 #define UA_GLOBAL
+#define UA_LDS
 #else
 #define UA_GLOBAL __attribute__((address_space(1)))
+#define UA_LDS __attribute__((address_space(3)))
 #endif
 template <class T>
 __device__ __forceinline__ UA_GLOBAL T *gptr(T *p) {
