@@ -286,7 +286,9 @@ __global__ __launch_bounds__(256) void k_atomic_probe(uint32_t *a, uint64_t mask
 
 // Latency probe: one wave follows a random cycle of indices (a[i] holds the next index),
 // one dependent access a step: KIND 0 a workgroup-scope load, 1 a workgroup-scope
-// atomicMin that changes nothing, 2 the same at agent scope, 3 an agent-scope load.
+// atomicMin that changes nothing, 2 the same at agent scope, 3 an agent-scope load. The
+// atomicMin's operand, mask + 1, exceeds every entry but is a run-time value: a constant
+// all-ones operand would let the compiler turn the atomic into a load.
 // out[0] = s_memrealtime ticks (100 MHz) of `iters` steps, out[1] the last index.
 template <int KIND>
 __global__ __launch_bounds__(64) void k_latency_probe(uint64_t *a, uint64_t mask, int iters, uint64_t *out) {
@@ -295,8 +297,8 @@ __global__ __launch_bounds__(64) void k_latency_probe(uint64_t *a, uint64_t mask
     for (int i = 0; i < iters; ++i) {
         uint64_t v;
         if (KIND == 0) v = __hip_atomic_load(a + idx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        else if (KIND == 1) v = __hip_atomic_fetch_min(a + idx, ~0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        else if (KIND == 2) v = __hip_atomic_fetch_min(a + idx, ~0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        else if (KIND == 1) v = __hip_atomic_fetch_min(a + idx, mask + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        else if (KIND == 2) v = __hip_atomic_fetch_min(a + idx, mask + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         else v = __hip_atomic_load(a + idx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         idx = v & mask;
     }

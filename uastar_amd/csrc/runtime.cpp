@@ -1159,13 +1159,16 @@ extern "C" int ua_debug_access_latency(ua_ctx *c, int log2_words, int kind, int 
     if (!c || !ns || log2_words < 6 || log2_words > 28 || kind < 0 || kind > 3 || iters < 1)
         return fail(c, UA_ERR_INVALID, "bad arguments");
     const uint64_t n = 1ull << log2_words;
+    // one random cycle through all n entries (Sattolo's shuffle of the identity, driven by
+    // Marsaglia's xorshift64), so that a chase never settles into a short, cached loop
     std::vector<uint64_t> h(n);
-    uint64_t x = 88172645463325252ull;  // xorshift64 (Marsaglia): an in-range index per entry
-    for (uint64_t i = 0; i < n; ++i) {
+    for (uint64_t i = 0; i < n; ++i) h[i] = i;
+    uint64_t x = 88172645463325252ull;
+    for (uint64_t i = n - 1; i > 0; --i) {
         x ^= x << 13;
         x ^= x >> 7;
         x ^= x << 17;
-        h[i] = x & (n - 1);
+        std::swap(h[i], h[x % i]);
     }
     DevBuf a, out;
     HIPCHK(c, a.ensure(n * 8));
