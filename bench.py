@@ -448,6 +448,13 @@ def run_serve_mixed(args):
         # hash values, hash-claiming CAS), counted by the kernel, over the serving launches
         "atomics": {"total": atomics, "per_launch": atomics / max(1, launches),
                     "per_s": atomics / (kms * 1e-3) if kms > 0 else 0.0},
+        # absolute search throughput: nodes expanded by the served queries over the
+        # serving span (wall) and over the serving launches' kernel time, and by the
+        # plain pass over its summed per-query service time
+        "expanded": {"serve_total": sum(r["expanded"] for r in sres),
+                     "serve_per_s_wall": sum(r["expanded"] for r in sres) / (span * 1e-3),
+                     "serve_per_s_kernel": sum(r["expanded"] for r in sres) / (kms * 1e-3) if kms > 0 else 0.0,
+                     "plain_per_s": sum(r["expanded"] for r in pres) / (sum(svc) * 1e-3)},
     }
     # the device's random-atomic throughput (ua_debug_atomic_rate, after the timed passes),
     # the ceiling of the relaxations
