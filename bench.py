@@ -547,6 +547,191 @@ def run_deep_grid(args):
     return out
 
 
+# ------------------------------------------------------------------ 15-puzzles plain, and the batch
+
+def cpu_puzzle_times(pdb, rows, budget_s, threads, puzzle_cap=4_000_000, seed=13):
+    """Single-thread oracle A* times (7-8 PDB) of a seeded random sample of fixture rows,
+    one puzzle per host thread, sized to about budget_s seconds on `threads` threads. A
+    puzzle beyond puzzle_cap expansions runs to puzzle_cap and its time is scaled by its
+    fixture expansion count. Returns (times, extrapolated count, wall seconds)."""
+    from oracle import oracle as O
+    rng = np.random.default_rng(seed)
+    order = [int(i) for i in rng.permutation(len(rows))]
+    sample, work = [], 0.0
+    for i in order:
+        if len(sample) >= threads and work >= budget_s * threads:
+            break
+        sample.append(i)
+        work += 3.5e-6 * min(rows[i]["expanded"], puzzle_cap)
+
+    def solve(i):
+        r = rows[i]
+        capped = r["expanded"] > puzzle_cap
+        t = time.perf_counter()
+        O.puzzle_solve(4, np.array(r["board"], np.uint8), pdb, "astar", node_limit=puzzle_cap if capped else 0)
+        dt = time.perf_counter() - t
+        return (dt * r["expanded"] / puzzle_cap, True) if capped else (dt, False)
+
+    t = time.perf_counter()
+    with cf.ThreadPoolExecutor(threads) as ex:
+        out = list(ex.map(solve, sample))
+    return [x[0] for x in out], sum(x[1] for x in out), time.perf_counter() - t
+
+
+def device_pdb4(ctx):
+    """The device-built 7-8 PDB, checked against the fixture digests (the CPU BFS takes minutes)."""
+    import hashlib
+    pdb = ctx.puzzle_download_pdb(4)
+    with open(os.path.join(ROOT, "tests", "golden", "pdb_digests.json")) as f:
+        digests = json.load(f)["4"]
+    off = 0
+    for d in digests:
+        assert hashlib.sha256(pdb[off: off + d["size"]].tobytes()).hexdigest() == d["sha256"], "PDB digest"
+        off += d["size"]
+    return pdb
+
+
+def run_plain_puzzle(args):
+    """The 256 random solvable 15-puzzles of seeds 3000..3255 (the fixture's boards and
+    CPU A* move counts), one plain whole-GPU launch each; `steps` of them are timed after
+    `warmup` others."""
+    import uastar_amd as U
+    with open(GOLDEN) as f:
+        rows = json.load(f)
+    n = max(1, min(args.steps, len(rows)))
+    timed, warm = rows[:n], rows[n:n + args.warmup] or rows[:1]
+    ctx = U.Context(0, graphs=args.graph, plain_heaps=args.plain_heaps)
+    ctx.puzzle_build_pdb(4)
+    for r in warm:
+        ctx.puzzle_solve(4, np.array(r["board"], np.uint8))
+    say(f"timing {n} plain 15-puzzles")
+    lat, res = [], []
+    t0 = time.perf_counter()
+    for r in timed:
+        t = time.perf_counter()
+        res.append(ctx.puzzle_solve(4, np.array(r["board"], np.uint8)))
+        lat.append((time.perf_counter() - t) * 1e3)
+    wall = time.perf_counter() - t0
+    mismatch = sum(not x["found"] or x["steps"] != r["steps"] for x, r in zip(res, timed))
+    alg = [puzzle_bytes_alg(x) for x in res]
+    kms = [x["kernel_ms"] for x in res]
+    achieved = sum(alg) / (sum(kms) * 1e-3) / 1e9
+    out = {
+        "metric": "queries/s + p50/p99 latency, batched serving vs plain GA*, 1 MI355X, exact",
+        "value": n / wall, "unit": "queries/s", "n_gpus": 1, "steps": n, "warmup": len(warm),
+        "ms_per_step": wall / n * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "int64", "data": "synthetic (seeded generators)",
+        "config": {"workload": f"plain GA*: {n} random solvable 15-puzzles (seeds 3000..{3000 + n - 1}), "
+                               "7-8 PDB built on the device, one query per whole-GPU launch",
+                   "mode": "plain", "heaps": sorted({x["heaps"] for x in res})},
+        "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
+        "exact_mismatches_vs_cpu_astar": mismatch,
+        "errors": sum(x["status"] < 0 for x in res),
+        "expanded_per_s": sum(x["expanded"] for x in res) / wall,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_teams<0, 0> (plain solve)",
+                     "kernel_ms_avg": float(np.mean(kms)), "alg_bytes_avg": float(np.mean(alg))},
+    }
+    say(f"GPU: {out['value']:.2f} queries/s, p50 {out['p50_ms']:.2f} ms, {mismatch} mismatches")
+    if not args.no_cpu:
+        say("CPU baseline (oracle A*, bounded sample)")
+        threads = min(16, os.cpu_count() or 1)
+        times, n_ext, cwall = cpu_puzzle_times(device_pdb4(ctx), timed, 20.0, threads)
+        model, nproc = host_cpu()
+        out["cpu_baseline"] = {
+            "value": threads / float(np.mean(times)), "unit": "queries/s", "cores": threads, "kind": "port",
+            "nproc": nproc, "cpu_model": model,
+            "sample": f"seeded random sample of {len(times)} of the {n} timed puzzles, solved by the oracle's A* "
+                      f"(7-8 PDB, reference containers) one per thread on {threads} threads in {cwall:.1f} s; "
+                      f"value = {threads} threads / mean single-thread time ({np.mean(times) * 1e3:.0f} ms); "
+                      f"{n_ext} time(s) extrapolated from 4M expansions by the fixture's A* expansion counts"}
+    ctx.close()
+    return out
+
+
+def run_batch_c4(args):
+    """The batched configuration: 4096 grids (side log-uniform in [256, 4096], p ~ U(0.1,
+    0.4), random open endpoints) plus the 512 random solvable 15-puzzles of seeds
+    41000..41511, solved by one ua_batch_solve call (persistent launches over
+    memory-bounded sub-batches). The same batch runs once untimed first: its device
+    arena (the budget's worth of HBM, allocated at about 40 GB/s) stays for the timed
+    call; the first call's wall time is reported beside."""
+    import uastar_amd as U
+    from tests.helpers import c4_grid_specs
+    gspecs = c4_grid_specs()
+    with open(os.path.join(ROOT, "tests", "golden", "c4_puzzles.json")) as f:
+        prows = json.load(f)
+    ctx = U.Context(0, max_rounds=1_000_000, graphs=args.graph)
+    ctx.puzzle_build_pdb(4)
+    say("materialising the batch")
+    grids = [ctx.grid_generate(s["H"], s["W"], s["seed"], s["p"], *s["s"], *s["g"]) for s in gspecs]
+    queries = [ctx.grid_query(g, *s["s"], *s["g"]) for g, s in zip(grids, gspecs)]
+    queries += [ctx.puzzle_query(4, r["board"]) for r in prows]
+    say(f"first (untimed) call: {len(queries)} queries")
+    t = time.perf_counter()
+    ctx.batch_solve(queries)
+    first = time.perf_counter() - t
+    say(f"timing one batch of {len(queries)} queries")
+    t = time.perf_counter()
+    res = ctx.batch_solve(queries)
+    wall = time.perf_counter() - t
+    gres, pres = res[:len(gspecs)], res[len(gspecs):]
+    mismatch = sum(not x["found"] or x["steps"] != r["steps"] for x, r in zip(pres, prows))
+    cells = sum(s["H"] * s["W"] for s in gspecs)
+    alg = sum(grid_bytes_alg(x) for x in gres) + sum(puzzle_bytes_alg(x) for x in pres)
+    out = {
+        "metric": "queries/s + p50/p99 latency, batched serving vs plain GA*, 1 MI355X, exact",
+        "value": len(queries) / wall, "unit": "queries/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": wall * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "int64", "data": "synthetic (seeded generators)",
+        "config": {"workload": f"one batch of {len(gspecs)} grids ({cells / 1e9:.2f} G cells, side log-uniform in "
+                               f"[256, 4096], p ~ U(0.1, 0.4), random open endpoints, seed stream 4) plus "
+                               f"{len(prows)} random solvable 15-puzzles (seeds 41000..41511)",
+                   "mode": "batch (persistent launches over memory-bounded sub-batches)"},
+        "puzzle_mismatches_vs_cpu_astar": mismatch,
+        "grids_found": sum(x["found"] for x in gres),
+        "errors": sum(x["status"] < 0 for x in res),
+        "expanded_per_s": sum(x["expanded"] for x in res) / wall,
+        "algorithmic_GBps": alg / wall / 1e9,
+        "first_call_s": first,
+    }
+    say(f"GPU: {out['value']:.1f} queries/s ({wall:.2f} s for the batch), {mismatch} puzzle mismatches")
+    if not args.no_cpu:
+        from oracle import oracle as O
+        say("CPU baseline (oracle, bounded sample, extrapolated to the batch)")
+        threads = min(16, os.cpu_count() or 1)
+        rng = np.random.default_rng(17)
+        gi = [int(i) for i in rng.permutation(len(gspecs))[:2 * threads]]
+
+        def gsolve(i):
+            s = gspecs[i]
+            g = O.grid_bernoulli(s["H"], s["W"], s["seed"], s["p"], *s["s"], *s["g"])
+            t = time.perf_counter()
+            O.grid_astar(g, s["H"], s["W"], *s["s"], *s["g"], "f32", want_path=False)
+            return time.perf_counter() - t
+
+        t = time.perf_counter()
+        with cf.ThreadPoolExecutor(threads) as ex:
+            gt = list(ex.map(gsolve, gi))
+        gwall = time.perf_counter() - t
+        pt, n_ext, pwall = cpu_puzzle_times(device_pdb4(ctx), prows, 10.0, threads)
+        est = float(np.mean(gt)) * len(gspecs) + float(np.mean(pt)) * len(prows)  # single-thread seconds
+        model, nproc = host_cpu()
+        out["cpu_baseline"] = {
+            "value": len(queries) / (est / threads), "unit": "queries/s", "cores": threads, "kind": "port",
+            "nproc": nproc, "cpu_model": model,
+            "sample": f"seeded random samples of {len(gi)} grids ({gwall:.1f} s) and {len(pt)} puzzles ({pwall:.1f} s) "
+                      f"of the batch, solved by the oracle (FP32 grid costs, reference containers; A* with the 7-8 PDB) "
+                      f"one per thread on {threads} threads; the batch's single-thread time extrapolated from the "
+                      f"samples' means ({est:.0f} s), value = queries / (that time / {threads} threads); {n_ext} puzzle "
+                      f"time(s) extrapolated from 4M expansions",
+            "extrapolated_batch_s": est / threads}
+    for g in grids:
+        g.free()
+    ctx.close()
+    return out
+
+
 # ------------------------------------------------------------------ main
 
 def main():
@@ -555,7 +740,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20,
                     help="serving stream: batches of %d arrivals; plain workloads: queries" % QUERIES_PER_STEP)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="serve-mixed", choices=["serve-mixed", "plain-grid1000", "plain-grid16384", "deep-grid"])
+    ap.add_argument("--workload", default="serve-mixed",
+                    choices=["serve-mixed", "plain-grid1000", "plain-grid16384", "deep-grid", "plain-puzzle", "batch-c4"])
     ap.add_argument("--load", type=float, default=20.0, help="arrival rate as a multiple of plain capacity")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default settings)")
@@ -598,6 +784,12 @@ def main():
         sys.exit(1)
     if args.workload == "serve-mixed":
         print(json.dumps(run_serve_mixed(args)))
+        return
+    if args.workload == "plain-puzzle":
+        print(json.dumps(run_plain_puzzle(args)))
+        return
+    if args.workload == "batch-c4":
+        print(json.dumps(run_batch_c4(args)))
         return
     if args.workload == "deep-grid":
         print(json.dumps(run_deep_grid(args)))
