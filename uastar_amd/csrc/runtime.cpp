@@ -1075,15 +1075,25 @@ extern "C" int ua_batch_solve(ua_ctx *c, const ua_query *qs, int64_t nq, ua_resu
             }
             pos = end;
         }
-        for (int64_t i : retry) {  // replan bigger
+        for (int64_t i : retry) {  // replan bigger: grids 4x the open list, puzzles 16x the hash
             scale[i] *= 4;
             if (descs[i].kind == KIND_GRID) {
                 const ua_query &u = qs[i];
                 grid_fill_desc(c, u.grid, u.sx, u.sy, u.ex, u.ey, descs[i].heaps, scale[i], descs[i],
                                plans[i]);
             } else {
-                int hl = descs[i].hash_log2 + 2;
-                puzzle_fill_desc(c, qs[i].n, qs[i].tiles, descs[i].heaps, hl, descs[i], plans[i]);
+                // a puzzle that filled its hash is a heavy one: 16x at once spares the second
+                // retry that 4x often needed (each retry reruns the search from its start),
+                // and a team 4x as wide, up to a quarter of the device, since a retry launch
+                // holds few queries
+                int hl = std::min(descs[i].hash_log2 + 4, kPuzzleSlotBits);
+                int per_cu = 1;
+                if (teams_max_blocks(&per_cu) != hipSuccess || per_cu < 1) per_cu = 1;
+                const int quarter = std::max(kBlockWaves, c->num_cu * per_cu * kBlockWaves / 4 / kBlockWaves * kBlockWaves);
+                const int heaps = std::max(descs[i].heaps, std::min(4 * descs[i].heaps, quarter));
+                const int cls = descs[i].cls;
+                puzzle_fill_desc(c, qs[i].n, qs[i].tiles, heaps, hl, descs[i], plans[i]);
+                descs[i].cls = cls;
             }
         }
         todo.swap(retry);
