@@ -134,3 +134,12 @@ def test_memory_bounded_scheme(c5):
     row = [r for r in json.load(open(os.path.join(GOLDEN, "c3_puzzles.json"))) if r["expanded"] < 300_000][0]
     r = c5.puzzle_solve_bounded(4, np.array(row["board"], np.uint8), 1 << 30)
     assert r["status"] == 0 and r["optimal"] == 1 and r["steps"] == row["steps"]
+    # and the hardest fixture 15-puzzle in 32 MiB: the plan (16-byte hash slots, heaps)
+    # keeps to the bound, and the weighted search returns a valid, longer-or-equal solution
+    hard4 = max(json.load(open(os.path.join(GOLDEN, "c3_puzzles.json"))), key=lambda r: r["expanded"])
+    _say(f"bounded 15-puzzle seed {hard4['seed']} in 32 MiB")
+    r = c5.puzzle_solve_bounded(4, np.array(hard4["board"], np.uint8), 32 << 20)
+    assert r["status"] == 0 and r["found"], r
+    assert r["steps"] >= hard4["steps"] and (r["steps"] - hard4["steps"]) % 2 == 0
+    mv = c5.puzzle_moves()
+    assert len(mv) == r["steps"] and replay_moves(4, hard4["board"], mv) == goal_board(4)

@@ -693,8 +693,9 @@ extern "C" int ua_puzzle_solve_bounded(ua_ctx *c, int n, const uint8_t *tiles, u
     }
     int rc = ensure_pdb(c, n);
     if (rc) return rc;
-    // half the bound for the hash (keys and values), half for the heaps
-    const size_t slot_bytes = n == 5 ? 20 : 12;
+    // half the bound for the hash (keys and values), half for the heaps; a slot is 16
+    // bytes (board, value) for N = 3, 4 and 20 for N = 5 (two board words, a value)
+    const size_t slot_bytes = n == 5 ? 20 : 16;
     int hl = 10;
     while (hl < 32 && (slot_bytes << (hl + 1)) <= mem_bytes / 2) ++hl;
     for (int ws = 0; ws <= 3; ++ws) {
@@ -713,6 +714,9 @@ extern "C" int ua_puzzle_solve_bounded(ua_ctx *c, int n, const uint8_t *tiles, u
         puzzle_fill_desc(c, n, tiles, heaps, hl, q, p);
         q.heap_cap = (int)std::min<int64_t>(per_heap + 1, 1 << 30);
         p.heap_bytes = (size_t)heaps * (size_t)(q.heap_cap - 1) * 512;
+        if (p.total() > mem_bytes)  // the plan's hash and heaps must keep to the bound
+            return fail(c, UA_ERR_INVALID, "bounded plan of %zu bytes exceeds the bound of %llu", p.total(),
+                        (unsigned long long)mem_bytes);
         q.wshift = ws;
         q.bounded = ws > 0;
         rc = run_wide(c, q, p);
