@@ -667,7 +667,9 @@ extern "C" int ua_puzzle_solve(ua_ctx *c, int n, const uint8_t *tiles, ua_result
     if (rc) return rc;
     int heaps = plain_heaps(c, KIND_PUZZLE);
     int hl = plain_hash_log2(c, n);
-    for (int tries = 0; tries < 4; ++tries, hl += 2) {
+    // a puzzle that fills its hash is a heavy one: each retry (a rerun from the start) gets
+    // 16x the slots, which spares the second retry that 4x often needed
+    for (int tries = 0; tries < 4; ++tries, hl = std::min(hl + 4, kPuzzleSlotBits)) {
         QueryDesc q;
         Plan p;
         puzzle_fill_desc(c, n, tiles, heaps, hl, q, p);
