@@ -1117,10 +1117,9 @@ __device__ __forceinline__ void team_solve(const QueryDesc &q, QueryResult *res,
 // This is the round of uAstar's kExtractExpand / kDeduplicate / kHeapInsert
 // (pathway/GPU-kernel.cuh:169-544) with its termination test on the device every round.
 //
-// The counters of a one-block team live in LDS (copied to the resume record when a
-// serving launch suspends the query); those of a team of several blocks live at the head
-// of the query's bucket storage and are exchanged at agent scope across the generation
-// barrier.
+// The ring runs on one-block teams only. Its counters live in LDS (copied to the resume
+// record when a serving launch suspends the query); the first kBucketMetaWords words of
+// the query's bucket storage are reserved for them and stay unused.
 struct BucketMeta {
     uint32_t tail[kBuckets];
     uint32_t head[2][kBuckets];  // consumed counts: round r reads head[r & 1], writes head[(r + 1) & 1]
