@@ -2,6 +2,9 @@
 // search itself (rounds, teams, grid and puzzle expansion) is in search.hpp.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cstdint>
+
 #include "common.hpp"
 #include "kernels.hpp"
 #include "search.hpp"
@@ -133,13 +136,28 @@ __global__ void k_grid_gen(uint64_t *bm, int H, int W, int wpr, uint64_t seed_mi
     bm[w] = word;
 }
 
-// Clears the open bit in the best word of every blocked cell of a table just filled
-// with all ones, so that each word carries its cell's open bit. One thread per cell.
-__global__ void k_grid_mark_blocked(uint64_t *best, const uint64_t *bm, int H, int W, int wpr) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (size_t)H * W) return;
-    int x = (int)(i / (unsigned)W), y = (int)(i - (size_t)x * W);
-    if (!((bm[(size_t)x * wpr + (y >> 6)] >> (y & 63)) & 1ull)) best[i] = ~kOpenBit;
+// Writes the best table of a grid before its search, each word once: all ones for an
+// open cell, all ones but the open bit for a blocked one, so that every word carries its
+// cell's open bit. Thread t of block (bx, by) takes the cell pair y = 512 bx + 2t, y + 1
+// in rows by, by + gridDim.y, ...: both cells lie in one bitmap word (y is even), and a
+// pair at an even index of a 16-byte aligned table is one 16-byte store.
+__global__ void k_grid_init_best(uint64_t *best, const uint64_t *bm, int H, int W, int wpr, int pair_store) {
+    const int y = (int)(blockIdx.x * blockDim.x + threadIdx.x) * 2;
+    if (y >= W) return;
+    for (int x = (int)blockIdx.y; x < H; x += (int)gridDim.y) {
+        const uint64_t bits = bm[(size_t)x * wpr + (y >> 6)] >> (y & 63);
+        const uint64_t v0 = (bits & 1) ? ~0ull : ~kOpenBit;
+        const uint64_t v1 = (bits & 2) ? ~0ull : ~kOpenBit;
+        const size_t i = (size_t)x * W + y;
+        if (y + 1 >= W) {
+            best[i] = v0;
+        } else if (pair_store && !(i & 1)) {
+            *reinterpret_cast<ulonglong2 *>(best + i) = make_ulonglong2(v0, v1);
+        } else {
+            best[i] = v0;
+            best[i + 1] = v1;
+        }
+    }
 }
 
 // Byte-per-cell input (pathway.hpp:116-119) to the bitmap. Each byte is a cell's mask of
@@ -340,9 +358,11 @@ hipError_t launch_grid_pack(const uint8_t *bytes, uint64_t *bm, int H, int W, in
     return hipGetLastError();
 }
 
-hipError_t launch_grid_mark_blocked(uint64_t *best, const uint64_t *bm, int H, int W, int wpr, hipStream_t st) {
-    size_t n = (size_t)H * W;
-    hipLaunchKernelGGL(k_grid_mark_blocked, dim3(grid_blocks(n, 256)), dim3(256), 0, st, best, bm, H, W, wpr);
+hipError_t launch_grid_init_best(uint64_t *best, const uint64_t *bm, int H, int W, int wpr, hipStream_t st) {
+    if (H <= 0 || W <= 0) return hipSuccess;
+    const dim3 grid((unsigned)((W + 511) / 512), (unsigned)std::min(H, 65535));
+    const int pair_store = ((uintptr_t)best & 15) == 0;
+    hipLaunchKernelGGL(k_grid_init_best, grid, dim3(256), 0, st, best, bm, H, W, wpr, pair_store);
     return hipGetLastError();
 }
 

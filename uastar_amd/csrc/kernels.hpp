@@ -11,8 +11,9 @@ hipError_t launch_grid_gen(uint64_t *bm, int H, int W, int wpr, uint64_t seed, u
 // if any byte is a partial out-move mask.
 hipError_t launch_grid_pack(const uint8_t *bytes, uint64_t *bm, int H, int W, int wpr, uint32_t *bad,
                             hipStream_t st);
-// After a fill of a grid best table with all ones: clears the open bit of blocked cells.
-hipError_t launch_grid_mark_blocked(uint64_t *best, const uint64_t *bm, int H, int W, int wpr, hipStream_t st);
+// Writes a grid's best table (H * W words) before its search: all ones for an open cell,
+// all ones but the open bit for a blocked one.
+hipError_t launch_grid_init_best(uint64_t *best, const uint64_t *bm, int H, int W, int wpr, hipStream_t st);
 // The search kernel (k_teams): `blocks` blocks of kBlockWaves wavefronts, cooperative,
 // over the I/O block io (StepHeader, then TeamVars, descriptors and results). mode
 // names the caller in profiles: 0 plain solve, 1 batch, 2 serving step; set is the

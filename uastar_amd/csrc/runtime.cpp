@@ -268,8 +268,7 @@ static int run_wide(ua_ctx *c, QueryDesc &q, const Plan &p) {
     q.vals = c->vals.as<uint32_t>();
     q.heap = c->heap.as<uint64_t>();
     if (q.kind == KIND_GRID) {
-        HIPCHK(c, hipMemsetAsync(q.best, 0xFF, p.best_bytes, c->st));
-        HIPCHK(c, launch_grid_mark_blocked(q.best, q.bitmap, q.H, q.W, q.wpr, c->st));
+        HIPCHK(c, launch_grid_init_best(q.best, q.bitmap, q.H, q.W, q.wpr, c->st));
     } else {
         HIPCHK(c, hipMemsetAsync(q.best, 0, p.best_bytes, c->st));
         if (p.vals_bytes) HIPCHK(c, hipMemsetAsync(q.vals, 0xFF, p.vals_bytes, c->st));
@@ -567,8 +566,7 @@ extern "C" int ua_grid_solve_noalloc(ua_ctx *c, const ua_grid *g, int sx, int sy
     q.bwin = std::max(1, std::min(c->tune.grid_bucket_window, kBuckets - 48));
     q.bkt_log2 = kNoAllocBktLog2;
     q.heap_cap = 0;
-    HIPCHK(c, hipMemsetAsync(q.best, 0xFF, (size_t)cells * 8, c->st));
-    HIPCHK(c, launch_grid_mark_blocked(q.best, q.bitmap, q.H, q.W, q.wpr, c->st));
+    HIPCHK(c, launch_grid_init_best(q.best, q.bitmap, q.H, q.W, q.wpr, c->st));
     // the I/O block of one one-block group: header, TeamVars (zero), descriptor, result
     const size_t tv_off = align256(sizeof(StepHeader));
     const size_t qd_off = tv_off + align256(sizeof(TeamVars));
@@ -1037,7 +1035,8 @@ extern "C" int ua_batch_solve(ua_ctx *c, const ua_query *qs, int64_t nq, ua_resu
             held += arena.n;
             int arena_idx = (int)c->b_arenas.size() - 1;
             char *base = arena.as<char>();
-            // layout: [grid best tables | 5x5 puzzle vals] filled with 0xFF, then [puzzle slots] = 0, then heaps
+            // layout: [grid best tables] (written by the init kernel), [5x5 puzzle vals] filled
+            // with 0xFF, [puzzle slots] = 0, then heaps
             std::vector<QueryDesc> sub;
             size_t off = 0;
             for (size_t k = pos; k < end; ++k) {
@@ -1046,11 +1045,16 @@ extern "C" int ua_batch_solve(ua_ctx *c, const ua_query *qs, int64_t nq, ua_resu
                 if (q.kind == KIND_GRID) {
                     q.best = reinterpret_cast<uint64_t *>(base + off);
                     off += align256(plans[i].best_bytes);
-                } else {
-                    q.vals = reinterpret_cast<uint32_t *>(base + off);
-                    off += align256(plans[i].vals_bytes);
                 }
                 sub.push_back(q);
+            }
+            const size_t ff_begin = off;
+            for (size_t k = pos; k < end; ++k) {
+                int64_t i = todo[k];
+                if (descs[i].kind == KIND_PUZZLE) {
+                    sub[k - pos].vals = reinterpret_cast<uint32_t *>(base + off);
+                    off += align256(plans[i].vals_bytes);
+                }
             }
             size_t ff_end = off;
             for (size_t k = pos; k < end; ++k) {
@@ -1065,10 +1069,10 @@ extern "C" int ua_batch_solve(ua_ctx *c, const ua_query *qs, int64_t nq, ua_resu
                 sub[k - pos].heap = reinterpret_cast<uint64_t *>(base + off);
                 off += align256(plans[todo[k]].heap_bytes);
             }
-            HIPCHK(c, hipMemsetAsync(base, 0xFF, ff_end, c->st));
-            if (zero_end > ff_end) HIPCHK(c, hipMemsetAsync(base + ff_end, 0, zero_end - ff_end, c->st));
             for (const QueryDesc &q : sub)
-                if (q.kind == KIND_GRID) HIPCHK(c, launch_grid_mark_blocked(q.best, q.bitmap, q.H, q.W, q.wpr, c->st));
+                if (q.kind == KIND_GRID) HIPCHK(c, launch_grid_init_best(q.best, q.bitmap, q.H, q.W, q.wpr, c->st));
+            if (ff_end > ff_begin) HIPCHK(c, hipMemsetAsync(base + ff_begin, 0xFF, ff_end - ff_begin, c->st));
+            if (zero_end > ff_end) HIPCHK(c, hipMemsetAsync(base + ff_end, 0, zero_end - ff_end, c->st));
             std::vector<QueryResult> sr;
             int rc = run_batch(c, sub, sr, 1);
             if (rc) return rc;

@@ -178,8 +178,7 @@ int plan_job(ua_ctx *c, ServeEngine *e, Job &j, bool *fits) {
     // (5x5: keys = 0, vals = all ones); no resume record
     hipError_t he;
     if (j.d.kind == KIND_GRID) {
-        he = hipMemsetAsync(j.d.best, 0xFF, j.plan.best_bytes, c->st);
-        if (he == hipSuccess) he = launch_grid_mark_blocked(j.d.best, j.d.bitmap, j.d.H, j.d.W, j.d.wpr, c->st);
+        he = launch_grid_init_best(j.d.best, j.d.bitmap, j.d.H, j.d.W, j.d.wpr, c->st);
     } else {
         he = hipMemsetAsync(j.d.best, 0, j.plan.best_bytes, c->st);
         if (he == hipSuccess && j.plan.vals_bytes) he = hipMemsetAsync(j.d.vals, 0xFF, j.plan.vals_bytes, c->st);
