@@ -1,6 +1,7 @@
 """Benchmark of the GA* solve path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload plain-grid1000|plain-grid16384|deep-grid]
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+                    [--workload plain-grid1000|plain-grid16384|deep-grid|plain-puzzle|plain-puzzle5|batch-c4]
     python bench.py --copy-calib 1024     # one 1 GiB device-to-device copy (counter calibration)
 
 Prints ONE JSON line (see the keys below). On the serving stream (the default workload)
@@ -549,11 +550,11 @@ def run_deep_grid(args):
 
 # ------------------------------------------------------------------ 15-puzzles plain, and the batch
 
-def cpu_puzzle_times(pdb, rows, budget_s, threads, puzzle_cap=4_000_000, seed=13):
-    """Single-thread oracle A* times (7-8 PDB) of a seeded random sample of fixture rows,
-    one puzzle per host thread, sized to about budget_s seconds on `threads` threads. A
-    puzzle beyond puzzle_cap expansions runs to puzzle_cap and its time is scaled by its
-    fixture expansion count. Returns (times, extrapolated count, wall seconds)."""
+def cpu_puzzle_times(pdb, rows, budget_s, threads, puzzle_cap=4_000_000, seed=13, n=4):
+    """Single-thread oracle A* times (the PDB of side n) of a seeded random sample of
+    fixture rows, one puzzle per host thread, sized to about budget_s seconds on `threads`
+    threads. A puzzle beyond puzzle_cap expansions runs to puzzle_cap and its time is
+    scaled by its fixture expansion count. Returns (times, extrapolated count, wall seconds)."""
     from oracle import oracle as O
     rng = np.random.default_rng(seed)
     order = [int(i) for i in rng.permutation(len(rows))]
@@ -568,7 +569,7 @@ def cpu_puzzle_times(pdb, rows, budget_s, threads, puzzle_cap=4_000_000, seed=13
         r = rows[i]
         capped = r["expanded"] > puzzle_cap
         t = time.perf_counter()
-        O.puzzle_solve(4, np.array(r["board"], np.uint8), pdb, "astar", node_limit=puzzle_cap if capped else 0)
+        O.puzzle_solve(n, np.array(r["board"], np.uint8), pdb, "astar", node_limit=puzzle_cap if capped else 0)
         dt = time.perf_counter() - t
         return (dt * r["expanded"] / puzzle_cap, True) if capped else (dt, False)
 
@@ -578,12 +579,13 @@ def cpu_puzzle_times(pdb, rows, budget_s, threads, puzzle_cap=4_000_000, seed=13
     return [x[0] for x in out], sum(x[1] for x in out), time.perf_counter() - t
 
 
-def device_pdb4(ctx):
-    """The device-built 7-8 PDB, checked against the fixture digests (the CPU BFS takes minutes)."""
+def device_pdb(ctx, n=4):
+    """The device-built PDB of side n (7-8 for the 15-puzzle, 6-6-6-6 for the 24-puzzle),
+    checked against the fixture digests (the CPU BFS takes minutes)."""
     import hashlib
-    pdb = ctx.puzzle_download_pdb(4)
+    pdb = ctx.puzzle_download_pdb(n)
     with open(os.path.join(ROOT, "tests", "golden", "pdb_digests.json")) as f:
-        digests = json.load(f)["4"]
+        digests = json.load(f)[str(n)]
     off = 0
     for d in digests:
         assert hashlib.sha256(pdb[off: off + d["size"]].tobytes()).hexdigest() == d["sha256"], "PDB digest"
@@ -636,7 +638,7 @@ def run_plain_puzzle(args):
     if not args.no_cpu:
         say("CPU baseline (oracle A*, bounded sample)")
         threads = min(16, os.cpu_count() or 1)
-        times, n_ext, cwall = cpu_puzzle_times(device_pdb4(ctx), timed, 20.0, threads)
+        times, n_ext, cwall = cpu_puzzle_times(device_pdb(ctx), timed, 20.0, threads)
         model, nproc = host_cpu()
         out["cpu_baseline"] = {
             "value": threads / float(np.mean(times)), "unit": "queries/s", "cores": threads, "kind": "port",
@@ -645,6 +647,70 @@ def run_plain_puzzle(args):
                       f"(7-8 PDB, reference containers) one per thread on {threads} threads in {cwall:.1f} s; "
                       f"value = {threads} threads / mean single-thread time ({np.mean(times) * 1e3:.0f} ms); "
                       f"{n_ext} time(s) extrapolated from 4M expansions by the fixture's A* expansion counts"}
+    ctx.close()
+    return out
+
+
+def run_plain_puzzle5(args):
+    """The 24-puzzle fixture boards (seeded walks off the goal, 22-68 moves, each solved
+    by the oracle's A* with the 6-6-6-6 PDB and by IDA* with Manhattan distance), one plain
+    whole-GPU launch each, after one warm-up solve. Every move count is checked against
+    the fixture."""
+    import uastar_amd as U
+    with open(os.path.join(ROOT, "tests", "golden", "c5_puzzles.json")) as f:
+        rows = sorted(json.load(f), key=lambda r: r["seed"])
+    n = max(1, min(args.steps, len(rows)))
+    timed = rows[:n]
+    warm = min(rows, key=lambda r: r["expanded"])
+    ctx = U.Context(0, graphs=args.graph, plain_heaps=args.plain_heaps)
+    say("building the 5x5 PDB on the device")
+    ctx.puzzle_build_pdb(5)
+    for _ in range(max(1, args.warmup)):
+        ctx.puzzle_solve(5, np.array(warm["board"], np.uint8))
+    say(f"timing {n} plain 24-puzzles")
+    lat, res = [], []
+    t0 = time.perf_counter()
+    for r in timed:
+        t = time.perf_counter()
+        res.append(ctx.puzzle_solve(5, np.array(r["board"], np.uint8)))
+        lat.append((time.perf_counter() - t) * 1e3)
+    wall = time.perf_counter() - t0
+    mismatch = sum(not x["found"] or x["steps"] != r["steps"] for x, r in zip(res, timed))
+    alg = [puzzle_bytes_alg(x) for x in res]
+    kms = [x["kernel_ms"] for x in res]
+    achieved = sum(alg) / (sum(kms) * 1e-3) / 1e9
+    out = {
+        "metric": "queries/s + p50/p99 latency, batched serving vs plain GA*, 1 MI355X, exact",
+        "value": n / wall, "unit": "queries/s", "n_gpus": 1, "steps": n, "warmup": max(1, args.warmup),
+        "ms_per_step": wall / n * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "int64", "data": "fixture boards (seeded walks off the goal)",
+        "config": {"workload": f"plain GA*: the {n} 24-puzzle fixture boards (seeds "
+                               f"{', '.join(str(r['seed']) for r in timed)}), 6-6-6-6 PDB built on the "
+                               "device, one query per whole-GPU launch",
+                   "mode": "plain", "heaps": sorted({x["heaps"] for x in res})},
+        "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
+        "per_query": [{"seed": r["seed"], "steps": x["steps"], "expanded": x["expanded"], "ms": round(t, 3)}
+                      for r, x, t in zip(timed, res, lat)],
+        "exact_mismatches_vs_cpu_astar": mismatch,
+        "errors": sum(x["status"] < 0 for x in res),
+        "expanded_per_s": sum(x["expanded"] for x in res) / wall,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_teams<0, 1> (plain solve, 5x5)",
+                     "kernel_ms_avg": float(np.mean(kms)), "alg_bytes_avg": float(np.mean(alg))},
+    }
+    say(f"GPU: {out['value']:.2f} queries/s, p50 {out['p50_ms']:.2f} ms, {mismatch} mismatches")
+    if not args.no_cpu:
+        say("CPU baseline (oracle A*, bounded sample)")
+        threads = min(16, os.cpu_count() or 1)
+        times, n_ext, cwall = cpu_puzzle_times(device_pdb(ctx, 5), timed, 20.0, threads, puzzle_cap=1_000_000, n=5)
+        model, nproc = host_cpu()
+        out["cpu_baseline"] = {
+            "value": threads / float(np.mean(times)), "unit": "queries/s", "cores": threads, "kind": "port",
+            "nproc": nproc, "cpu_model": model,
+            "sample": f"seeded random sample of {len(times)} of the {n} timed boards, solved by the oracle's A* "
+                      f"(6-6-6-6 PDB, reference containers) one per thread on {threads} threads in {cwall:.1f} s; "
+                      f"value = {threads} threads / mean single-thread time ({np.mean(times) * 1e3:.0f} ms); "
+                      f"{n_ext} time(s) extrapolated from 1M expansions by the fixture's A* expansion counts"}
     ctx.close()
     return out
 
@@ -714,7 +780,7 @@ def run_batch_c4(args):
         with cf.ThreadPoolExecutor(threads) as ex:
             gt = list(ex.map(gsolve, gi))
         gwall = time.perf_counter() - t
-        pt, n_ext, pwall = cpu_puzzle_times(device_pdb4(ctx), prows, 10.0, threads)
+        pt, n_ext, pwall = cpu_puzzle_times(device_pdb(ctx), prows, 10.0, threads)
         est = float(np.mean(gt)) * len(gspecs) + float(np.mean(pt)) * len(prows)  # single-thread seconds
         model, nproc = host_cpu()
         out["cpu_baseline"] = {
@@ -741,7 +807,8 @@ def main():
                     help="serving stream: batches of %d arrivals; plain workloads: queries" % QUERIES_PER_STEP)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="serve-mixed",
-                    choices=["serve-mixed", "plain-grid1000", "plain-grid16384", "deep-grid", "plain-puzzle", "batch-c4"])
+                    choices=["serve-mixed", "plain-grid1000", "plain-grid16384", "deep-grid", "plain-puzzle",
+                             "plain-puzzle5", "batch-c4"])
     ap.add_argument("--load", type=float, default=20.0, help="arrival rate as a multiple of plain capacity")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default settings)")
@@ -787,6 +854,9 @@ def main():
         return
     if args.workload == "plain-puzzle":
         print(json.dumps(run_plain_puzzle(args)))
+        return
+    if args.workload == "plain-puzzle5":
+        print(json.dumps(run_plain_puzzle5(args)))
         return
     if args.workload == "batch-c4":
         print(json.dumps(run_batch_c4(args)))
