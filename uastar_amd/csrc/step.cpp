@@ -201,10 +201,11 @@ int run_step(ua_ctx *c, int mode, int set, int blocks, const LaunchShape &shape,
         HIPCHK(c, hipEventRecord(c->ev1, c->st));
         HIPCHK(c, hipMemcpyAsync(c->h_io + l.qr_off, io + l.qr_off, l.end - l.qr_off, hipMemcpyDeviceToHost, c->st));
     }
-    if (mode == 2) {
-        // A serving step ends within its time budget: poll the stream rather than let the
-        // thread sleep until the runtime wakes it, which adds latency to every step. After
-        // 50 ms of polling (a step never runs that long) the thread waits normally.
+    if (mode != 1) {
+        // A serving step ends within its time budget, and most plain solves take a few
+        // milliseconds: poll the stream rather than let the thread sleep until the runtime
+        // wakes it, which adds latency to every launch. After 50 ms of polling (a serving
+        // step never runs that long) the thread waits normally.
         const auto spin_until = Clock::now() + std::chrono::milliseconds(50);
         hipError_t q;
         while ((q = hipStreamQuery(c->st)) == hipErrorNotReady && Clock::now() < spin_until) {
