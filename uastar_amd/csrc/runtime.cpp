@@ -667,16 +667,18 @@ extern "C" int ua_puzzle_solve(ua_ctx *c, int n, const uint8_t *tiles, ua_result
     int hl = plain_hash_log2(c, n);
     // a puzzle that fills its hash is a heavy one: each retry (a rerun from the start) gets
     // 16x the slots, which spares the second retry that 4x often needed
+    double kms = 0;  // every attempt's launch
     for (int tries = 0; tries < 4; ++tries, hl = std::min(hl + 4, kPuzzleSlotBits)) {
         QueryDesc q;
         Plan p;
-        puzzle_fill_desc(c, n, tiles, heaps, hl, q, p);
+        if ((rc = puzzle_fill_desc(c, n, tiles, heaps, hl, q, p))) return rc;
         rc = run_wide(c, q, p);
         if (rc) return rc;
+        kms += c->last_kernel_ms;
         if (c->last_r.status != ST_CAPACITY) break;
     }
     fill_result(c->last_q, c->last_r, res);
-    res->kernel_ms = c->last_kernel_ms;
+    res->kernel_ms = kms;
     if (res->status < 0) fail(c, res->status, "puzzle solve failed with status %d", res->status);
     return res->status;
 }
@@ -711,7 +713,7 @@ extern "C" int ua_puzzle_solve_bounded(ua_ctx *c, int n, const uint8_t *tiles, u
         if (per_heap < 3) return fail(c, UA_ERR_INVALID, "memory bound too small for %d heaps", heaps);
         QueryDesc q;
         Plan p;
-        puzzle_fill_desc(c, n, tiles, heaps, hl, q, p);
+        if ((rc = puzzle_fill_desc(c, n, tiles, heaps, hl, q, p))) return rc;
         q.heap_cap = (int)std::min<int64_t>(per_heap + 1, 1 << 30);
         p.heap_bytes = (size_t)heaps * (size_t)(q.heap_cap - 1) * 512;
         if (p.total() > mem_bytes)  // the plan's hash and heaps must keep to the bound
