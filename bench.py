@@ -1,7 +1,8 @@
 """Benchmark of the GA* solve path on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
-                    [--workload plain-grid1000|plain-grid16384|deep-grid|plain-puzzle|plain-puzzle5|batch-c4]
+                    [--workload plain-grid1000|plain-grid16384|deep-grid|plain-puzzle|plain-puzzle5|
+                                bounded-puzzle5|batch-c4]
     python bench.py --copy-calib 1024     # one 1 GiB device-to-device copy (counter calibration)
 
 Prints ONE JSON line (see the keys below). On the serving stream (the default workload)
@@ -715,6 +716,90 @@ def run_plain_puzzle5(args):
     return out
 
 
+def replay_board(n, board, moves):
+    """The board after the blank moves (0..3 = down, up, right, left), or None if a move
+    leaves the board."""
+    b = [int(t) for t in board]
+    z = b.index(0)
+    for m in moves:
+        x, y = z // n + (1, -1, 0, 0)[m], z % n + (0, 0, 1, -1)[m]
+        if not (0 <= x < n and 0 <= y < n):
+            return None
+        b[z], b[x * n + y] = b[x * n + y], b[z]
+        z = x * n + y
+    return b
+
+
+def run_bounded_puzzle5(args):
+    """The memory-bounded scheme on the three hardest 24-puzzle fixture boards (56, 64 and
+    68 moves; 3.0-7.0 M A* expansions on the CPU) under bounds from 4 GiB down to 64 MiB:
+    plain A* where it fits (an optimal answer), else weighted searches in the same memory.
+    Every returned move list is replayed to the goal and its length set against the
+    fixture's optimum."""
+    import uastar_amd as U
+    with open(os.path.join(ROOT, "tests", "golden", "c5_puzzles.json")) as f:
+        rows = sorted(json.load(f), key=lambda r: -r["expanded"])[:3]
+    bounds = [4 << 30, 1 << 30, 256 << 20, 64 << 20]
+    goal = list(range(1, 25)) + [0]
+    ctx = U.Context(0, graphs=args.graph, plain_heaps=args.plain_heaps)
+    say("building the 5x5 PDB on the device")
+    ctx.puzzle_build_pdb(5)
+    ctx.puzzle_solve_bounded(5, np.array(min(rows, key=lambda r: r["expanded"])["board"], np.uint8), bounds[0])
+    say(f"timing {len(rows)} boards x {len(bounds)} memory bounds")
+    solves, bad = [], 0
+    t0 = time.perf_counter()
+    for bound in bounds:
+        for r in rows:
+            t = time.perf_counter()
+            x = ctx.puzzle_solve_bounded(5, np.array(r["board"], np.uint8), bound)
+            ms = (time.perf_counter() - t) * 1e3
+            mv = ctx.puzzle_moves() if x["found"] else []
+            ok = bool(x["found"]) and len(mv) == x["steps"] and replay_board(5, r["board"], mv) == goal
+            ok = ok and x["steps"] >= r["steps"] and (not x["optimal"] or x["steps"] == r["steps"])
+            bad += not ok
+            solves.append({"seed": r["seed"], "bound_mib": bound >> 20, "steps": x["steps"], "optimum": r["steps"],
+                           "optimal": x["optimal"], "weight": x["weight"], "expanded": x["expanded"],
+                           "kernel_ms": round(x["kernel_ms"], 3), "ms": round(ms, 3),
+                           "alg_bytes": puzzle_bytes_alg(x)})
+    wall = time.perf_counter() - t0
+    kms = sum(s["kernel_ms"] for s in solves)
+    achieved = sum(s["alg_bytes"] for s in solves) / (kms * 1e-3) / 1e9
+    out = {
+        "metric": "queries/s + p50/p99 latency, batched serving vs plain GA*, 1 MI355X, exact",
+        "value": len(solves) / wall, "unit": "queries/s", "n_gpus": 1, "steps": len(solves), "warmup": 1,
+        "ms_per_step": wall / len(solves) * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "int64", "data": "fixture boards (seeded walks off the goal)",
+        "config": {"workload": "memory-bounded scheme: the 24-puzzle fixture boards of seeds "
+                               f"{', '.join(str(r['seed']) for r in rows)} under bounds of "
+                               f"{', '.join(str(b >> 20) for b in bounds)} MiB, one solve each",
+                   "mode": "plain (bounded)"},
+        "p50_ms": pct([s["ms"] for s in solves], 50), "p99_ms": pct([s["ms"] for s in solves], 99),
+        "solves": solves,
+        "invalid_solutions": bad,
+        "errors": 0,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_teams<0, 1> (plain and weighted solves, 5x5)",
+                     "kernel_ms_avg": kms / len(solves),
+                     "alg_bytes_avg": float(np.mean([s["alg_bytes"] for s in solves]))},
+    }
+    say(f"GPU: {out['value']:.2f} solves/s, {bad} invalid")
+    if not args.no_cpu:
+        say("CPU baseline (oracle A*, unbounded, bounded sample)")
+        threads = min(16, os.cpu_count() or 1)
+        times, n_ext, cwall = cpu_puzzle_times(device_pdb(ctx, 5), rows, 10.0, threads, puzzle_cap=1_000_000, n=5)
+        model, nproc = host_cpu()
+        out["cpu_baseline"] = {
+            "value": threads / float(np.mean(times)), "unit": "queries/s", "cores": threads, "kind": "port",
+            "nproc": nproc, "cpu_model": model,
+            "sample": f"the oracle's plain A* (6-6-6-6 PDB, reference containers, no memory bound: the reference "
+                      f"has no bounded scheme) on the {len(times)} boards, one per thread, in {cwall:.1f} s; "
+                      f"value = {threads} threads / mean single-thread time ({np.mean(times) * 1e3:.0f} ms); "
+                      f"{n_ext} time(s) extrapolated from 1M expansions by the fixture's A* expansion counts"}
+    ctx.close()
+    return out
+
+
 def run_batch_c4(args):
     """The batched configuration: 4096 grids (side log-uniform in [256, 4096], p ~ U(0.1,
     0.4), random open endpoints) plus the 512 random solvable 15-puzzles of seeds
@@ -808,7 +893,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="serve-mixed",
                     choices=["serve-mixed", "plain-grid1000", "plain-grid16384", "deep-grid", "plain-puzzle",
-                             "plain-puzzle5", "batch-c4"])
+                             "plain-puzzle5", "bounded-puzzle5", "batch-c4"])
     ap.add_argument("--load", type=float, default=20.0, help="arrival rate as a multiple of plain capacity")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default settings)")
@@ -857,6 +942,9 @@ def main():
         return
     if args.workload == "plain-puzzle5":
         print(json.dumps(run_plain_puzzle5(args)))
+        return
+    if args.workload == "bounded-puzzle5":
+        print(json.dumps(run_bounded_puzzle5(args)))
         return
     if args.workload == "batch-c4":
         print(json.dumps(run_batch_c4(args)))

@@ -76,9 +76,9 @@ typedef struct {
     int64_t rounds, pops, expanded, generated, inserted;
     double device_ms;    /* search time on the device (s_memrealtime stamps of the team) */
     double kernel_ms;    /* duration of the search launch, from HIP events on the context's stream
-                            (a plain puzzle solve that reran with a larger hash: every attempt's
-                            launch; a batch: the launch that ran this query, shared by its
-                            sub-batch) */
+                            (a plain puzzle solve that reran with a larger hash, or a bounded
+                            solve that reran weighted: every attempt's launch; a batch: the
+                            launch that ran this query, shared by its sub-batch) */
     double latency_ms;   /* serve: completion - submission; batch: completion - launch */
     int32_t optimal;     /* 1: the cost is proven optimal (every exact search that finds a path) */
     int32_t weight;      /* weight on h of the search that found the answer (1 = plain A*) */

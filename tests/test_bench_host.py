@@ -53,3 +53,16 @@ def test_fifo_latency_of_the_plain_pass():
     # arrivals 0, 1, 10 ms with service 5, 5, 1 ms: the second waits for the first
     # (done at 5, then 10), the third arrives to an idle server
     assert bench.fifo_latency([0.0, 1.0, 10.0], [5.0, 5.0, 1.0]) == [5.0, 9.0, 1.0]
+
+
+def test_move_replay_against_the_fixture_solutions():
+    """bench.replay_board (the check of the bounded workload's answers) takes every stored
+    24-puzzle solution to the goal, and a solution with one move dropped does not."""
+    with open(os.path.join(ROOT, "tests", "golden", "c5_puzzles.json")) as f:
+        rows = json.load(f)
+    goal = list(range(1, 25)) + [0]
+    for r in rows:
+        assert len(r["moves"]) == r["steps"]
+        assert bench.replay_board(5, r["board"], r["moves"]) == goal, r["seed"]
+        if r["moves"]:
+            assert bench.replay_board(5, r["board"], r["moves"][:-1]) != goal

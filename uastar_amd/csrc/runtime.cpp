@@ -698,6 +698,7 @@ extern "C" int ua_puzzle_solve_bounded(ua_ctx *c, int n, const uint8_t *tiles, u
     // half the bound for the hash (keys and values), half for the heaps; a slot is 16
     // bytes (board, value) for N = 3, 4 and 20 for N = 5 (two board words, a value)
     const size_t slot_bytes = n == 5 ? 20 : 16;
+    double kms = 0;  // every attempt's launch
     int hl = 10;
     while (hl < 32 && (slot_bytes << (hl + 1)) <= mem_bytes / 2) ++hl;
     for (int ws = 0; ws <= 3; ++ws) {
@@ -723,12 +724,13 @@ extern "C" int ua_puzzle_solve_bounded(ua_ctx *c, int n, const uint8_t *tiles, u
         q.bounded = ws > 0;
         rc = run_wide(c, q, p);
         if (rc) return rc;
+        kms += c->last_kernel_ms;
         const QueryResult &r = c->last_r;
         if (r.status == ST_OK && r.found) break;
         if (ws == 0 && r.status != ST_CAPACITY && r.status != ST_RANGE) break;  // no path, or an error
     }
     fill_result(c->last_q, c->last_r, res);
-    res->kernel_ms = c->last_kernel_ms;
+    res->kernel_ms = kms;
     if (res->status == UA_NO_PATH && c->last_q.bounded) res->status = UA_ERR_CAPACITY;  // pruned away
     if (res->status < 0) fail(c, res->status, "bounded puzzle solve failed with status %d", res->status);
     return res->status;
