@@ -382,12 +382,12 @@ struct GridProb {
                 if (!is_goal && !(w[d] & kOpenBit)) continue;
                 ++k.gen;
                 const uint64_t ng = gk + (d < 4 ? KEY_ONE : KEY_SQRT2);
+                if ((w[d] >> kGridKeyShift) <= ng) continue;  // no key: its f is not needed
                 const uint64_t nf = ng + oct(uabs(gx + kDX[d]), uabs(gy + kDY[d]));
                 if (nf >= KEY_LIMIT) {
                     err |= FLAG_RANGE;
                     continue;
                 }
-                if ((w[d] >> kGridKeyShift) <= ng) continue;
                 ++k.atom;
                 old[j] = sc.fmin(bp + (kDX[d] * Wl + kDY[d]), (ng << kGridKeyShift) | (w[d] & kOpenBit) | (uint64_t)d);
                 if (!is_goal)
