@@ -26,8 +26,8 @@ def bctx():
 
 
 @pytest.mark.skip(reason="hung twice on the GPU with the former workgroup-team batch kernel (k_batch, removed); "
-                         "cause not shown: the SIMT emulator runs the same inputs correctly. Kept off the GPU for "
-                         "this round; the one-block-group batch path is covered by test_gpu_teams_batch.py")
+                         "cause not shown: the SIMT emulator runs the same inputs correctly. Kept off the GPU until "
+                         "the cause is shown; the one-block-group batch path is covered by test_gpu_teams_batch.py")
 def test_mixed_batch_matches_oracle(bctx, oracle):
     ctx = bctx
     rng = np.random.default_rng(42)
@@ -67,8 +67,8 @@ def test_mixed_batch_matches_oracle(bctx, oracle):
 
 
 @pytest.mark.skip(reason="hung twice on the GPU with the former workgroup-team batch kernel (k_batch, removed); "
-                         "cause not shown: the SIMT emulator runs the same inputs correctly. Kept off the GPU for "
-                         "this round; the one-block-group batch path is covered by test_gpu_teams_batch.py")
+                         "cause not shown: the SIMT emulator runs the same inputs correctly. Kept off the GPU until "
+                         "the cause is shown; the one-block-group batch path is covered by test_gpu_teams_batch.py")
 def test_alone_equals_in_mix(bctx, oracle):
     ctx = bctx
     """The same query gives the same cost alone, in a small batch and in a large mixed batch."""
